@@ -1,0 +1,148 @@
+// Shared device helpers for the CDNA4 (gfx950, wave64) kernels of the FUGAL
+// alignment pipeline, plus host-side error plumbing for the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstddef>
+#include <cstdint>
+
+namespace cg {
+
+constexpr int kWave = 64;
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLn2 = 0.69314718055994531f;
+// Finite stand-in for -inf in running maxima: keeps (x - m) free of inf-inf.
+constexpr float kNegBig = -FLT_MAX;
+
+// v_exp_f32 / v_log_f32: base-2 hardware transcendentals (about 1 ulp).
+// Results below 2^-126 may flush to zero; every sum below adds terms that
+// are <= 1 relative to the running maximum, so a flushed term changes a
+// log-sum-exp by at most n * 2^-126 relative.
+__device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float log2_fast(float x) { return __builtin_amdgcn_logf(x); }
+
+// Streaming loads that bypass cache retention (data read once per pass).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_stream4(const float* p) {
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+
+// Running (max, scaled sum) pair of a base-2 log-sum-exp.
+struct Lse {
+    float m;
+    float s;
+};
+
+// Merge two (max, sum) pairs.  A pair with s == 0 is the identity.
+__device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
+    const float m = fmaxf(a.m, b.m);
+    const float s = a.s * exp2_fast(a.m - m) + b.s * exp2_fast(b.m - m);
+    return {m, s};
+}
+
+// Online update of (m, s) with one term x: one exp2 per term.
+__device__ __forceinline__ void lse_push(float& m, float& s, float x) {
+    const float d = x - m;
+    const float e = exp2_fast(-fabsf(d));
+    const bool up = d > 0.0f;
+    s = up ? fmaf(s, e, 1.0f) : s + e;
+    m = up ? x : m;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Wave-wide log-sum-exp of per-lane pairs: max first, then one rescale per
+// lane and a plain sum, so only one exp2 per lane.
+__device__ __forceinline__ Lse wave_lse(Lse p) {
+    const float m = wave_max(p.m);
+    const float s = wave_sum(p.s * exp2_fast(p.m - m));
+    return {m, s};
+}
+
+// Block-wide double sum; `scratch` needs blockDim.x / 64 doubles.  Result is
+// valid in thread 0 only.  Summation order is fixed, hence deterministic.
+__device__ __forceinline__ double block_sum_double(double v, double* scratch) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double total = 0.0;
+    if (threadIdx.x == 0) {
+        const int waves = blockDim.x / kWave;
+        for (int w = 0; w < waves; ++w) total += scratch[w];
+    }
+    __syncthreads();
+    return total;
+}
+
+__host__ __device__ constexpr int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ constexpr int64_t round_up(int64_t a, int64_t b) { return div_up(a, b) * b; }
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// ---------------------------------------------------------------------------
+// Host-side status handling shared by every C-ABI entry point.
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+// Checks hipGetLastError() after a launch; returns 0 or a negative code.
+int check_launch(const char* what);
+
+// Carve aligned sub-buffers out of a caller-provided workspace.
+struct Carver {
+    char* base;
+    size_t used;
+    size_t cap;
+    template <typename T>
+    T* take(size_t count) {
+        used = static_cast<size_t>(round_up(static_cast<int64_t>(used), 256));
+        T* p = reinterpret_cast<T*>(base ? base + used : nullptr);
+        used += count * sizeof(T);
+        return p;
+    }
+    bool ok() const { return used <= cap; }
+};
+
+}  // namespace cg
+
+// Status codes returned by every entry point (0 = success).
+#define CG_OK 0
+#define CG_EINVAL (-1)
+#define CG_EWORKSPACE (-2)
+#define CG_ELAUNCH (-3)
+#define CG_EUNSUPPORTED (-4)
+#define CG_EHIP (-5)
+
+#define CG_CHECK_HIP(expr)                                                        \
+    do {                                                                          \
+        hipError_t cg_err_ = (expr);                                              \
+        if (cg_err_ != hipSuccess)                                                \
+            return ::cg::fail(CG_EHIP, "%s failed: %s", #expr, hipGetErrorString(cg_err_)); \
+    } while (0)

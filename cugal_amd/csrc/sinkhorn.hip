@@ -1,0 +1,846 @@
+// Log-domain Sinkhorn-Knopp for the FUGAL Frank-Wolfe loop on CDNA4.
+//
+// Semantics follow cugal/sinkhorn.py:90-120 (solve_log): with K = -C/reg,
+//   v_j = -LSE_i(K_ij + u_i),   u_i = -LSE_j(K_ij + v_j)
+// per iteration, a column-marginal L1 error every `eval_freq` iterations
+// (not at iteration 0) and a stop when it drops below the threshold.  The
+// kernels work in base-2 units (u2 = u*log2(e), K2 = C*(-log2(e)/reg)), so
+// every exponential is one v_exp_f32.
+//
+// Memory traffic: C (n x n FP32, row-major) is never rescaled, transposed or
+// copied.  Column log-sum-exps are formed from per-row-block partial
+// (max, sum) pairs that a small combine kernel merges, so
+//   * the fused path reads C once per iteration: a row block computes u_i for
+//     each of its rows and, with the row still in registers, folds
+//     K2_ij + u2_i + v2_j into running column pairs (the next v and the
+//     column marginal of the current (u, v) both come out of the combine);
+//   * the two-sweep path reads C twice per iteration (column sweep, then row
+//     sweep), the same structure as the reference kernels
+//     (cuda_kernels/sinkhorn_log.cu:39-66 applied to K^T and K).
+// Convergence is decided on the device: the last block of a checking combine
+// records the error and raises `done`; every later kernel of the solve exits
+// at entry, so the host only polls once per check period and never stalls
+// the queue.
+#include "common.hpp"
+#include "cugal_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace cg {
+namespace {
+
+struct SinkCtl {
+    int done;        // 1 once the threshold is met
+    int iterations;  // iteration count at the stop (valid when done)
+    int final_buf;   // v buffer holding the final v (valid when done)
+    int nerr;        // errors recorded so far
+    unsigned ticket; // arrival counter of the combine's blocks
+    float momentum;  // current over-relaxation factor (1 = plain Sinkhorn)
+    float pad0;
+    int pad1;
+    // followed by `double errors[capacity]`
+};
+
+__device__ __forceinline__ double* ctl_errors(SinkCtl* c) {
+    return reinterpret_cast<double*>(reinterpret_cast<char*>(c) + sizeof(SinkCtl));
+}
+
+// ---------------------------------------------------------------------------
+// Set-up and tear-down.
+// ---------------------------------------------------------------------------
+__global__ void sk_init(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ u2,
+                        float* __restrict__ v2, int n, SinkCtl* __restrict__ ctl) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        u2[j] = u[j] * kLog2e;
+        v2[j] = v[j] * kLog2e;
+    }
+    if (j == 0) {
+        ctl->done = 0;
+        ctl->iterations = 0;
+        ctl->final_buf = 0;
+        ctl->nerr = 0;
+        ctl->ticket = 0u;
+        ctl->momentum = 1.0f;
+    }
+}
+
+__global__ void sk_finish(const float* __restrict__ u2, const float* __restrict__ v2a,
+                          const float* __restrict__ v2b, float* __restrict__ u, float* __restrict__ v, int n,
+                          const SinkCtl* __restrict__ ctl, int default_buf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int buf = ctl->done ? ctl->final_buf : default_buf;
+    u[j] = u2[j] * kLn2;
+    v[j] = (buf ? v2b[j] : v2a[j]) * kLn2;
+}
+
+// ---------------------------------------------------------------------------
+// Fused sweep: one read of C per iteration.
+//
+// Block b owns rows [b*rpb, (b+1)*rpb) and every column.  Thread t owns the
+// float4 column groups g*T + t (g < G), i.e. 4G columns, so a row load is one
+// coalesced float4 per group.  The current v sits in LDS.  Per row:
+//   x_j  = K2_ij + v2_j                         (registers)
+//   u2_i = (1-m) u2_i - m * LSE2_j(x_j)         (one block reduction)
+//   s_j += 2^(x_j + u2_i)                       (= the plan entry)
+// The column sums need no running maximum: the sweep always follows a
+// v-update, after which every column of the plan sums to 1; the row
+// normalisation then leaves every entry <= 1 and every column sum >= 1/n
+// (sum_i r_i = n with row sums r_i <= n).  So s_j never overflows, and terms
+// flushed below 2^-126 change a column sum by at most n * 2^-126 / (1/n)
+// relative.  (With Lehmann over-relaxation, factor <= 2, the bounds widen to
+// n^2, still far inside FP32.)  The next row is loaded while the current one
+// is reduced (two register buffers).
+// ---------------------------------------------------------------------------
+template <int T, int G>
+__device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __restrict__ row, int n) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + threadIdx.x) * 4;
+        r[g] = col < n ? ld_stream4(row + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+template <int T, int G>
+__device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], int i, int n, float scale2,
+                                          const float* __restrict__ sv, float2* __restrict__ red,
+                                          float* __restrict__ u2, float mom) {
+    constexpr int W = T / kWave;
+    const int tid = threadIdx.x;
+    const float u_old = u2[i];
+    float lm = kNegBig;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (col < n) {
+            const float4 vv = *reinterpret_cast<const float4*>(sv + col);
+            r[g].x = fmaf(r[g].x, scale2, vv.x);
+            r[g].y = fmaf(r[g].y, scale2, vv.y);
+            r[g].z = fmaf(r[g].z, scale2, vv.z);
+            r[g].w = fmaf(r[g].w, scale2, vv.w);
+            lm = fmaxf(lm, fmaxf(fmaxf(r[g].x, r[g].y), fmaxf(r[g].z, r[g].w)));
+        }
+    }
+    float ls = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if ((g * T + tid) * 4 < n)
+            ls += (exp2_fast(r[g].x - lm) + exp2_fast(r[g].y - lm)) + (exp2_fast(r[g].z - lm) + exp2_fast(r[g].w - lm));
+    }
+    const Lse w = wave_lse({lm, ls});
+    float2* slot = red + (i & 1) * W;
+    if ((tid & (kWave - 1)) == 0) slot[tid / kWave] = make_float2(w.m, w.s);
+    __syncthreads();
+    float M = kNegBig;
+#pragma unroll
+    for (int k = 0; k < W; ++k) M = fmaxf(M, slot[k].x);
+    float S = 0.0f;
+#pragma unroll
+    for (int k = 0; k < W; ++k) S += slot[k].y * exp2_fast(slot[k].x - M);
+    const float lse = M + log2_fast(S);
+    const float ui = (mom == 1.0f) ? -lse : (1.0f - mom) * u_old - mom * lse;
+    if (tid == 0) u2[i] = ui;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if ((g * T + tid) * 4 < n) {
+            cs[4 * g + 0] += exp2_fast(r[g].x + ui);
+            cs[4 * g + 1] += exp2_fast(r[g].y + ui);
+            cs[4 * g + 2] += exp2_fast(r[g].z + ui);
+            cs[4 * g + 3] += exp2_fast(r[g].w + ui);
+        }
+    }
+}
+
+template <int T, int G>
+__global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C, int n, float scale2,
+                                                   const float* __restrict__ v2, float* __restrict__ u2,
+                                                   float* __restrict__ part, int rows_per_blk,
+                                                   const SinkCtl* __restrict__ ctl) {
+    extern __shared__ float smem[];
+    if (ctl->done) return;
+    const float mom = ctl->momentum;
+    const int tid = threadIdx.x;
+    float* sv = smem;
+    float2* red = reinterpret_cast<float2*>(smem + round_up(n, 4));
+    for (int j = tid * 4; j < n; j += T * 4)
+        *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v2 + j);
+    __syncthreads();
+
+    float cs[4 * G];
+#pragma unroll
+    for (int k = 0; k < 4 * G; ++k) cs[k] = 0.0f;
+
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    const int64_t ld = n;
+    if constexpr (4 * G * T <= 24 * 1024) {
+        // Two register buffers: row i+1 is in flight while row i is reduced.
+        float4 a[G], b[G];
+        if (r0 < r1) fused_load_row<T, G>(a, C + r0 * ld, n);
+        for (int i = r0; i < r1; i += 2) {
+            if (i + 1 < r1) fused_load_row<T, G>(b, C + (i + 1) * ld, n);
+            fused_row<T, G>(a, cs, i, n, scale2, sv, red, u2, mom);
+            if (i + 1 >= r1) break;
+            if (i + 2 < r1) fused_load_row<T, G>(a, C + (i + 2) * ld, n);
+            fused_row<T, G>(b, cs, i + 1, n, scale2, sv, red, u2, mom);
+        }
+    } else {
+        // 32 columns per thread: a second row buffer would not fit in the 128
+        // registers a 1024-thread block allows; latency is hidden by the 16
+        // waves' outstanding loads instead.
+        float4 a[G];
+        for (int i = r0; i < r1; ++i) {
+            fused_load_row<T, G>(a, C + i * ld, n);
+            fused_row<T, G>(a, cs, i, n, scale2, sv, red, u2, mom);
+        }
+    }
+
+    float* out = part + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (col < n)
+            *reinterpret_cast<float4*>(out + col) = make_float4(cs[4 * g], cs[4 * g + 1], cs[4 * g + 2], cs[4 * g + 3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column sweep (two-sweep path, and any n): block (bx, by) folds rows
+// [bx*rpb, ...) of column chunk by into per-column pairs of K2 + u2_i + v2_j.
+// Scalar layout so any n and any row stride work.
+// ---------------------------------------------------------------------------
+template <int T, int CPT>
+__global__ __launch_bounds__(T) void sk_col_sweep(const float* __restrict__ C, int n, int64_t ldc, float scale2,
+                                                 const float* __restrict__ v2, const float* __restrict__ u2,
+                                                 float2* __restrict__ part, int rows_per_blk,
+                                                 const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int c0 = blockIdx.y * (T * CPT);
+    float vj[CPT], cm[CPT], cs[CPT];
+    bool ok[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int col = c0 + k * T + threadIdx.x;
+        ok[k] = col < n;
+        vj[k] = ok[k] ? v2[col] : 0.0f;
+        cm[k] = kNegBig;
+        cs[k] = 0.0f;
+    }
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    for (int i = r0; i < r1; ++i) {
+        const float* row = C + static_cast<int64_t>(i) * ldc;
+        const float ui = u2[i];
+        float c[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int col = c0 + k * T + threadIdx.x;
+            c[k] = ok[k] ? ld_stream(row + col) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+            if (ok[k]) lse_push(cm[k], cs[k], fmaf(c[k], scale2, vj[k]) + ui);
+    }
+    float2* out = part + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int col = c0 + k * T + threadIdx.x;
+        if (ok[k]) out[col] = make_float2(cm[k], cs[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row sweep: one wave per row,
+//   out_i = (1-m) out_i - m * oscale * LSE2_j(C_ij*cscale + add_j*ascale).
+// The solver calls it in base-2 units (oscale = ascale = 1); the reference-
+// compatible single step (cuda_kernels/sinkhorn_log.cu:39-66) calls it with
+// K in natural units (cscale = ascale = log2 e, oscale = ln 2).
+// ---------------------------------------------------------------------------
+template <int W, bool VEC>
+__global__ __launch_bounds__(W * kWave) void sk_row_sweep(const float* __restrict__ C, int n, int64_t ldc,
+                                                         float cscale, const float* __restrict__ add, float ascale,
+                                                         float oscale, float* __restrict__ out, float mom_arg,
+                                                         const SinkCtl* __restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const float mom = ctl ? ctl->momentum : mom_arg;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * W + threadIdx.x / kWave;
+    if (i >= n) return;
+    const float* row = C + i * ldc;
+    float m = kNegBig, s = 0.0f;
+    if (VEC) {
+        for (int j = lane * 4; j < n; j += kWave * 4) {
+            const float4 c = ld_stream4(row + j);
+            const float4 a = *reinterpret_cast<const float4*>(add + j);
+            const float x0 = fmaf(c.x, cscale, a.x * ascale);
+            const float x1 = fmaf(c.y, cscale, a.y * ascale);
+            const float x2 = fmaf(c.z, cscale, a.z * ascale);
+            const float x3 = fmaf(c.w, cscale, a.w * ascale);
+            const float mn = fmaxf(m, fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)));
+            s = s * exp2_fast(m - mn) + ((exp2_fast(x0 - mn) + exp2_fast(x1 - mn)) + (exp2_fast(x2 - mn) + exp2_fast(x3 - mn)));
+            m = mn;
+        }
+    } else {
+        for (int j = lane; j < n; j += kWave) lse_push(m, s, fmaf(row[j], cscale, add[j] * ascale));
+    }
+    const Lse w = wave_lse({m, s});
+    if (lane == 0) {
+        const float lse = (w.m + log2_fast(w.s)) * oscale;
+        out[i] = (mom == 1.0f) ? -lse : (1.0f - mom) * out[i] - mom * lse;
+    }
+}
+
+// Lehmann over-relaxation, cugal/sinkhorn.py:78-88, evaluated on the device
+// after `nerr` errors have been recorded.  Python's negative indexing of the
+// error list is kept.
+__device__ float lehmann_momentum(const double* errors, int nerr, int eval_freq, int momentum_start) {
+    if (momentum_start < 0 || nerr * eval_freq < momentum_start) return 1.0f;
+    const int index = momentum_start / eval_freq;
+    if (nerr < index - 1) return 1.0f;
+    const int a = index - 1 >= 0 ? index - 1 : nerr + index - 1;
+    const int b = index - 2 >= 0 ? index - 2 : nerr + index - 2;
+    if (a < 0 || b < 0 || a >= nerr || b >= nerr) return 1.0f;
+    double ratio = errors[a] / errors[b];
+    if (isnan(ratio)) ratio = 0.99;
+    ratio = fmin(ratio, 0.99);
+    const double power = 1.0 / eval_freq;
+    return static_cast<float>(2.0 / (1.0 + sqrt(1.0 - pow(ratio, power))));
+}
+
+// ---------------------------------------------------------------------------
+// Combine: merge the per-block column partials into
+//   L_j = log2 sum_i plan_ij   (plan of the current (u, v)),
+// then
+//   v2_next_j = v2_j - L_j                       (momentum 1), or
+//   raw LSE2_i(K2_ij + u2_i) = L_j - v2_j         (momentum active: the blend
+//                                                  runs after the last block
+//                                                  has updated m),
+// and the column marginal is 2^L_j.  PAIRS: (max, sum) partials of the exact
+// column sweep; otherwise plain sums of the fused sweep.  A block covers 64
+// columns; its 4 waves split the partial list.  With want_err the blocks'
+// FP64 error sums are totalled in fixed order by the last block to arrive.
+// ---------------------------------------------------------------------------
+constexpr int kCombCols = 64;
+constexpr int kCombWaves = 4;
+
+template <bool PAIRS>
+__global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
+    const void* __restrict__ part_raw, int nblk, int n, const float* __restrict__ v2_in, float* __restrict__ v2_out,
+    int raw_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
+    double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
+    __shared__ float2 red[kCombWaves][kCombCols];
+    __shared__ double dred[kCombWaves];
+    __shared__ int is_last;
+    if (ctl->done) return;
+    const int lane = threadIdx.x & (kCombCols - 1);
+    const int wave = threadIdx.x / kCombCols;
+    const int j = blockIdx.x * kCombCols + lane;
+    Lse a = {0.0f, 0.0f};
+    if (PAIRS) {
+        a.m = kNegBig;
+        const float2* part = static_cast<const float2*>(part_raw);
+        if (j < n)
+            for (int b = wave; b < nblk; b += kCombWaves) {
+                const float2 p = part[static_cast<int64_t>(b) * n + j];
+                a = lse_merge(a, {p.x, p.y});
+            }
+    } else {
+        const float* part = static_cast<const float*>(part_raw);
+        if (j < n)
+            for (int b = wave; b < nblk; b += kCombWaves) a.s += part[static_cast<int64_t>(b) * n + j];
+    }
+    red[wave][lane] = make_float2(a.m, a.s);
+    __syncthreads();
+    double err = 0.0;
+    if (wave == 0 && j < n) {
+        float L;
+        if (PAIRS) {
+            Lse t = {kNegBig, 0.0f};
+            for (int w = 0; w < kCombWaves; ++w) t = lse_merge(t, {red[w][lane].x, red[w][lane].y});
+            L = t.m + log2_fast(t.s);
+        } else {
+            float S = 0.0f;
+            for (int w = 0; w < kCombWaves; ++w) S += red[w][lane].y;
+            L = log2_fast(S);
+        }
+        const float vj = v2_in[j];
+        v2_out[j] = raw_out ? (L - vj) : (vj - L);
+        if (want_err) err = fabs(exp2(static_cast<double>(L)) - 1.0);
+    }
+    if (!want_err) return;
+    // Block sum of the error (wave 0 holds all terms).
+    if (wave == 0) {
+        err = wave_sum(err);
+        if (lane == 0) dred[0] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        errpart[blockIdx.x] = dred[0];
+        __threadfence();
+        const unsigned t = atomicAdd(&ctl->ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double total = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x) total += errpart[b];
+    // Fixed-shape reduction over the block: deterministic.
+    total = wave_sum(total);
+    if ((threadIdx.x & 63) == 0) dred[threadIdx.x / 64] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kCombWaves; ++w) sum += dred[w];
+        double* errors = ctl_errors(ctl);
+        errors[ctl->nerr] = sum;
+        ctl->nerr += 1;
+        ctl->ticket = 0u;
+        if (sum < threshold) {
+            ctl->done = 1;
+            ctl->iterations = check_index + 1;
+            ctl->final_buf = cur_buf;
+        } else {
+            ctl->momentum = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
+        }
+    }
+}
+
+// v_next = (1 - m) v - m * lse  (only when Lehmann momentum is configured).
+__global__ void sk_apply_momentum(const float* __restrict__ v2_in, float* __restrict__ v2_io, int n,
+                                  const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float m = ctl->momentum;
+    const float lse = v2_io[j];
+    v2_io[j] = (m == 1.0f) ? -lse : (1.0f - m) * v2_in[j] - m * lse;
+}
+
+// Row marginal error of a materialised kernel, cuda_kernels/sinkhorn_log.cu:68-98:
+// out_r = |sum_c exp(K_rc + v_c + u_r) - 1|.
+__global__ __launch_bounds__(256) void sk_row_marginal(const float* __restrict__ K, int n, const float* __restrict__ u,
+                                                      const float* __restrict__ v, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / 64;
+    if (r >= n) return;
+    const float ur = u[r];
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += exp2_fast((K[r * n + c] + v[c] + ur) * kLog2e);
+    s = wave_sum(s);
+    if (lane == 0) out[r] = fabs(s - 1.0);
+}
+
+__global__ void sum_doubles(const double* __restrict__ in, int n, double* __restrict__ out) {
+    __shared__ double scratch[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += in[i];
+    s = block_sum_double(s, scratch);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+constexpr int kFusedMaxGroups = 8;  // 1024 threads x 8 float4 groups = 32768 columns
+constexpr int64_t kFusedMaxN = 1024 * 4 * kFusedMaxGroups;
+constexpr int kColThreads = 256;
+constexpr int kColCPT = 8;
+constexpr int kRowWaves = 4;
+
+struct Plan {
+    int fused;
+    int threads;       // fused: block size
+    int groups;        // fused: float4 groups per thread
+    int rows_per_blk;  // fused: rows per block
+    int nblk;          // fused: row blocks (= partial rows)
+    int col_rows;      // column sweep: rows per block
+    int col_nblk;      // column sweep: row blocks
+    int col_chunks;    // column sweep: chunks of kColThreads*kColCPT columns
+    int comb_blocks;
+};
+
+Plan make_plan(int64_t n, int fused) {
+    Plan p{};
+    p.fused = fused;
+    if (fused) {
+        p.threads = n > 8192 ? 1024 : 256;
+        const int64_t g = div_up(n, static_cast<int64_t>(p.threads) * 4);
+        p.groups = g <= 1 ? 1 : g <= 2 ? 2 : g <= 4 ? 4 : g <= 6 ? 6 : 8;
+        // 1024-thread blocks: one per CU (256 CUs); 256-thread blocks: two.
+        // The column sums of a block cost 4n bytes, 1/rpb of C.
+        const int64_t target = p.threads == 1024 ? 256 : 512;
+        p.rows_per_blk = static_cast<int>(std::max<int64_t>(1, div_up(n, target)));
+        p.nblk = static_cast<int>(div_up(n, p.rows_per_blk));
+    }
+    p.col_chunks = static_cast<int>(div_up(n, static_cast<int64_t>(kColThreads) * kColCPT));
+    const int64_t col_target = std::max<int64_t>(1, 1024 / p.col_chunks);
+    p.col_rows = static_cast<int>(std::max<int64_t>(8, div_up(n, col_target)));
+    p.col_nblk = static_cast<int>(div_up(n, p.col_rows));
+    p.comb_blocks = static_cast<int>(div_up(n, kCombCols));
+    return p;
+}
+
+size_t ctl_bytes(int err_capacity) { return sizeof(SinkCtl) + sizeof(double) * static_cast<size_t>(err_capacity); }
+
+int err_capacity(int max_iterations, int eval_freq) { return max_iterations / std::max(1, eval_freq) + 4; }
+
+struct SinkWs {
+    float* u2;
+    float* v2[2];
+    void* part;
+    double* errpart;
+    SinkCtl* ctl;
+};
+
+size_t carve(Carver& cv, int64_t n, const Plan& p, int cap, SinkWs* w) {
+    SinkWs t{};
+    t.u2 = cv.take<float>(n);
+    t.v2[0] = cv.take<float>(n);
+    t.v2[1] = cv.take<float>(n);
+    const size_t pair_bytes = sizeof(float2) * static_cast<size_t>(p.col_nblk) * n;
+    const size_t sum_bytes = p.fused ? sizeof(float) * static_cast<size_t>(p.nblk) * n : 0;
+    t.part = cv.take<char>(std::max(pair_bytes, sum_bytes));
+    t.errpart = cv.take<double>(p.comb_blocks);
+    t.ctl = reinterpret_cast<SinkCtl*>(cv.take<char>(ctl_bytes(cap)));
+    if (w) *w = t;
+    return cv.used;
+}
+
+int max_lds_bytes() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        cached = v;
+    }
+    return cached;
+}
+
+template <int T, int G>
+int launch_fused_tg(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
+                    const SinkCtl* ctl, hipStream_t s) {
+    const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float2) * 2 * (T / kWave);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
+        attr_set = true;
+    }
+    if (lds > static_cast<size_t>(max_lds_bytes()))
+        return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
+                    max_lds_bytes());
+    hipLaunchKernelGGL((sk_fused_sweep<T, G>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
+                       p.rows_per_blk, ctl);
+    return check_launch("sk_fused_sweep");
+}
+
+int launch_fused(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
+                 const SinkCtl* ctl, hipStream_t s) {
+    if (p.threads == 256) {
+        switch (p.groups) {
+            case 1: return launch_fused_tg<256, 1>(C, n, scale2, v2, u2, part, p, ctl, s);
+            case 2: return launch_fused_tg<256, 2>(C, n, scale2, v2, u2, part, p, ctl, s);
+            case 4: return launch_fused_tg<256, 4>(C, n, scale2, v2, u2, part, p, ctl, s);
+            case 6: return launch_fused_tg<256, 6>(C, n, scale2, v2, u2, part, p, ctl, s);
+            default: return launch_fused_tg<256, 8>(C, n, scale2, v2, u2, part, p, ctl, s);
+        }
+    }
+    switch (p.groups) {
+        case 1: return launch_fused_tg<1024, 1>(C, n, scale2, v2, u2, part, p, ctl, s);
+        case 2: return launch_fused_tg<1024, 2>(C, n, scale2, v2, u2, part, p, ctl, s);
+        case 4: return launch_fused_tg<1024, 4>(C, n, scale2, v2, u2, part, p, ctl, s);
+        case 6: return launch_fused_tg<1024, 6>(C, n, scale2, v2, u2, part, p, ctl, s);
+        default: return launch_fused_tg<1024, 8>(C, n, scale2, v2, u2, part, p, ctl, s);
+    }
+}
+
+int launch_col(const float* C, int n, int64_t ldc, float scale2, const float* v2, const float* u2, float2* part,
+               const Plan& p, const SinkCtl* ctl, hipStream_t s) {
+    hipLaunchKernelGGL((sk_col_sweep<kColThreads, kColCPT>), dim3(p.col_nblk, p.col_chunks), dim3(kColThreads), 0, s,
+                       C, n, ldc, scale2, v2, u2, part, p.col_rows, ctl);
+    return check_launch("sk_col_sweep");
+}
+
+int launch_row(const float* C, int n, int64_t ldc, float cscale, const float* add, float ascale, float oscale,
+               float* out, float mom, const SinkCtl* ctl, hipStream_t s) {
+    const bool vec = (n % 4 == 0) && (ldc % 4 == 0) && aligned16(C) && aligned16(add);
+    const dim3 grid(static_cast<unsigned>(div_up(n, kRowWaves)));
+    if (vec)
+        hipLaunchKernelGGL((sk_row_sweep<kRowWaves, true>), grid, dim3(kRowWaves * kWave), 0, s, C, n, ldc, cscale,
+                           add, ascale, oscale, out, mom, ctl);
+    else
+        hipLaunchKernelGGL((sk_row_sweep<kRowWaves, false>), grid, dim3(kRowWaves * kWave), 0, s, C, n, ldc, cscale,
+                           add, ascale, oscale, out, mom, ctl);
+    return check_launch("sk_row_sweep");
+}
+
+// Pinned host mirror of the control block, grown on demand.
+struct HostMirror {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+HostMirror g_mirror;
+int* g_done_slots = nullptr;  // pinned poll slots
+
+int ensure_pinned(size_t bytes) {
+    if (!g_done_slots) CG_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_done_slots), 64 * sizeof(int), hipHostMallocDefault));
+    if (g_mirror.bytes >= bytes) return 0;
+    if (g_mirror.ptr) CG_CHECK_HIP(hipHostFree(g_mirror.ptr));
+    g_mirror.ptr = nullptr;
+    g_mirror.bytes = 0;
+    CG_CHECK_HIP(hipHostMalloc(&g_mirror.ptr, bytes, hipHostMallocDefault));
+    g_mirror.bytes = bytes;
+    return 0;
+}
+
+// Optional timing of the sweep kernels (bench roofline).
+struct SweepTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    int device = -1;
+};
+SweepTimer g_timer;
+
+int timer_begin(bool on) {
+    g_timer.on = on;
+    g_timer.used = 0;
+    if (!on) return 0;
+    int dev = 0;
+    CG_CHECK_HIP(hipGetDevice(&dev));
+    if (dev != g_timer.device) {
+        for (auto e : g_timer.ev) (void)hipEventDestroy(e);
+        g_timer.ev.clear();
+        g_timer.device = dev;
+    }
+    return 0;
+}
+
+int timer_mark(hipStream_t s) {
+    if (!g_timer.on) return 0;
+    if (g_timer.used == g_timer.ev.size()) {
+        hipEvent_t e;
+        CG_CHECK_HIP(hipEventCreate(&e));
+        g_timer.ev.push_back(e);
+    }
+    CG_CHECK_HIP(hipEventRecord(g_timer.ev[g_timer.used++], s));
+    return 0;
+}
+
+// Sum of (end - start) over recorded pairs; call after the stream is synced.
+int timer_total(float* ms, int* launches) {
+    float total = 0.0f;
+    int count = 0;
+    for (size_t k = 0; k + 1 < g_timer.used; k += 2) {
+        float t = 0.0f;
+        CG_CHECK_HIP(hipEventElapsedTime(&t, g_timer.ev[k], g_timer.ev[k + 1]));
+        total += t;
+        ++count;
+    }
+    if (ms) *ms = total;
+    if (launches) *launches = count;
+    return 0;
+}
+
+}  // namespace
+}  // namespace cg
+
+using namespace cg;
+
+extern "C" {
+
+size_t cg_sinkhorn_workspace(int64_t n, int fused, int max_iterations, int eval_freq) {
+    if (n <= 0) return 0;
+    const int use_fused = fused && cg_sinkhorn_fused_supported(n) && n % 4 == 0;
+    const Plan p = make_plan(n, use_fused);
+    Carver cv{nullptr, 0, ~size_t(0)};
+    return carve(cv, n, p, err_capacity(max_iterations, eval_freq), nullptr) + 256;
+}
+
+int cg_sinkhorn_fused_supported(int64_t n) {
+    return n > 0 && n <= kFusedMaxN ? 1 : 0;
+}
+
+int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float* u, float* v, int max_iterations,
+                          double threshold, int eval_freq, int momentum_start, int fused, double* errors,
+                          int errors_capacity, int* error_count, int* iterations, float* sweep_ms,
+                          int* sweep_launches, void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!C || !u || !v || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_sinkhorn_log_solve: bad arguments (n=%lld)", static_cast<long long>(n));
+    if (!(regularization > 0.0f)) return fail(CG_EINVAL, "sinkhorn regularization must be > 0");
+    if (eval_freq <= 0) return fail(CG_EINVAL, "sinkhorn eval_freq must be > 0");
+    if (max_iterations < 0) return fail(CG_EINVAL, "sinkhorn iterations must be >= 0");
+    const int ni = static_cast<int>(n);
+    const int use_fused = fused && cg_sinkhorn_fused_supported(n);
+    if (use_fused && (!aligned16(C) || n % 4 != 0)) {
+        // The fused sweep loads float4 groups: fall back to the two-sweep
+        // structure for unaligned or odd-width matrices (same results).
+        return cg_sinkhorn_log_solve(C, n, regularization, u, v, max_iterations, threshold, eval_freq,
+                                     momentum_start, 0, errors, errors_capacity, error_count, iterations, sweep_ms,
+                                     sweep_launches, workspace, workspace_bytes, stream);
+    }
+    const Plan p = make_plan(n, use_fused);
+    const int cap = err_capacity(max_iterations, eval_freq);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    SinkWs w{};
+    carve(cv, n, p, cap, &w);
+    if (!workspace || !cv.ok())
+        return fail(CG_EWORKSPACE, "cg_sinkhorn_log_solve: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    if (int rc = ensure_pinned(ctl_bytes(cap))) return rc;
+    if (int rc = timer_begin(sweep_ms != nullptr || sweep_launches != nullptr)) return rc;
+
+    const float scale2 = -kLog2e / regularization;
+    const int64_t ldc = n;
+    const bool momentum = momentum_start >= 0;
+    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
+
+    hipLaunchKernelGGL(sk_init, vgrid, dim3(256), 0, s, u, v, w.u2, w.v2[0], ni, w.ctl);
+    if (int rc = check_launch("sk_init")) return rc;
+
+    // Poll slots: one per outstanding check, host waits one period behind.
+    struct Pending {
+        hipEvent_t ev;
+        int slot;
+    };
+    std::vector<Pending> pending;
+    int next_slot = 0;
+    bool stopped = false;
+    auto post_check = [&]() -> int {
+        hipEvent_t ev;
+        CG_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const int slot = next_slot++ & 63;
+        CG_CHECK_HIP(hipMemcpyAsync(g_done_slots + slot, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        CG_CHECK_HIP(hipEventRecord(ev, s));
+        pending.push_back({ev, slot});
+        if (pending.size() >= 2) {
+            Pending old = pending.front();
+            pending.erase(pending.begin());
+            CG_CHECK_HIP(hipEventSynchronize(old.ev));
+            CG_CHECK_HIP(hipEventDestroy(old.ev));
+            if (g_done_slots[old.slot]) stopped = true;
+        }
+        return 0;
+    };
+    // pairs: partials of the exact column sweep; else sums of the fused sweep.
+    auto combine = [&](bool pairs, int cur, int want_err, int check_index) -> int {
+        if (pairs)
+            hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+                               p.col_nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
+                               threshold, eval_freq, momentum_start, w.errpart, w.ctl);
+        else
+            hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+                               p.nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
+                               threshold, eval_freq, momentum_start, w.errpart, w.ctl);
+        if (int rc = check_launch("sk_combine")) return rc;
+        if (momentum) {
+            hipLaunchKernelGGL(sk_apply_momentum, vgrid, dim3(256), 0, s, w.v2[cur], w.v2[cur ^ 1], ni, w.ctl);
+            if (int rc = check_launch("sk_apply_momentum")) return rc;
+        }
+        return 0;
+    };
+    auto is_check = [&](int it) { return it % eval_freq == 0 && it != 0; };
+
+    int cur = 0;
+    int final_default = 0;
+    if (max_iterations > 0) {
+        if (use_fused) {
+            // First v-update with the incoming u: exact column sweep (no bound
+            // on the plan holds before it).
+            if (int rc = launch_col(C, ni, ldc, scale2, w.v2[0], w.u2, static_cast<float2*>(w.part), p, w.ctl, s))
+                return rc;
+            if (int rc = combine(true, 0, 0, -1)) return rc;
+            cur = 1;
+            for (int it = 0; it < max_iterations && !stopped; ++it) {
+                const bool chk = is_check(it);
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = launch_fused(C, ni, scale2, w.v2[cur], w.u2, static_cast<float*>(w.part), p, w.ctl, s))
+                    return rc;
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = combine(false, cur, chk ? 1 : 0, it)) return rc;
+                final_default = cur;
+                cur ^= 1;
+                if (chk)
+                    if (int rc = post_check()) return rc;
+            }
+        } else {
+            // Two sweeps per iteration; the check of iteration it uses the
+            // column pairs of iteration it+1 (same u, same v).
+            for (int it = 0; it < max_iterations && !stopped; ++it) {
+                const bool prev_chk = it > 0 && is_check(it - 1);
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = launch_col(C, ni, ldc, scale2, w.v2[cur], w.u2, static_cast<float2*>(w.part), p, w.ctl, s))
+                    return rc;
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = combine(true, cur, prev_chk ? 1 : 0, it - 1)) return rc;
+                cur ^= 1;
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = launch_row(C, ni, ldc, scale2, w.v2[cur], 1.0f, 1.0f, w.u2, 1.0f, w.ctl, s)) return rc;
+                if (int rc = timer_mark(s)) return rc;
+                final_default = cur;
+                if (prev_chk)
+                    if (int rc = post_check()) return rc;
+            }
+            if (!stopped && is_check(max_iterations - 1)) {
+                // The last iteration is a checking one: one more column sweep
+                // to record its error (the returned state is unchanged).
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = launch_col(C, ni, ldc, scale2, w.v2[cur], w.u2, static_cast<float2*>(w.part), p, w.ctl, s))
+                    return rc;
+                if (int rc = timer_mark(s)) return rc;
+                if (int rc = combine(true, cur, 1, max_iterations - 1)) return rc;
+            }
+        }
+    }
+    hipLaunchKernelGGL(sk_finish, vgrid, dim3(256), 0, s, w.u2, w.v2[0], w.v2[1], u, v, ni, w.ctl, final_default);
+    if (int rc = check_launch("sk_finish")) return rc;
+    CG_CHECK_HIP(hipMemcpyAsync(g_mirror.ptr, w.ctl, ctl_bytes(cap), hipMemcpyDeviceToHost, s));
+    CG_CHECK_HIP(hipStreamSynchronize(s));
+    for (auto& pe : pending) (void)hipEventDestroy(pe.ev);
+    const SinkCtl* hc = static_cast<const SinkCtl*>(g_mirror.ptr);
+    const double* herr = reinterpret_cast<const double*>(static_cast<const char*>(g_mirror.ptr) + sizeof(SinkCtl));
+    if (iterations) *iterations = hc->done ? hc->iterations : max_iterations;
+    const int ne = std::min(hc->nerr, errors ? errors_capacity : 0);
+    if (errors) std::memcpy(errors, herr, sizeof(double) * static_cast<size_t>(std::max(0, ne)));
+    if (error_count) *error_count = hc->nerr;
+    if (int rc = timer_total(sweep_ms, sweep_launches)) return rc;
+    return CG_OK;
+}
+
+int cg_sinkhorn_log_step(const float* K, int64_t n, const float* add, float* out, float momentum,
+                         cg_stream_t stream) {
+    if (!K || !add || !out || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_sinkhorn_log_step: bad arguments");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const float mom = std::fabs(momentum - 1.0f) < 1e-4f ? 1.0f : momentum;
+    return launch_row(K, static_cast<int>(n), n, kLog2e, add, kLog2e, kLn2, out, mom, nullptr, s);
+}
+
+size_t cg_sinkhorn_log_marginal_workspace(int64_t n) { return n > 0 ? sizeof(double) * (n + 1) + 256 : 0; }
+
+int cg_sinkhorn_log_marginal(const float* K, int64_t n, const float* u, const float* v, double* result,
+                             void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    if (!K || !u || !v || !result || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_sinkhorn_log_marginal: bad arguments");
+    if (!workspace || workspace_bytes < cg_sinkhorn_log_marginal_workspace(n))
+        return fail(CG_EWORKSPACE, "cg_sinkhorn_log_marginal: workspace too small");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double* rows = static_cast<double*>(workspace);
+    double* total = rows + n;
+    hipLaunchKernelGGL(sk_row_marginal, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0, s, K,
+                       static_cast<int>(n), u, v, rows);
+    if (int rc = check_launch("sk_row_marginal")) return rc;
+    hipLaunchKernelGGL(sum_doubles, dim3(1), dim3(256), 0, s, rows, static_cast<int>(n), total);
+    if (int rc = check_launch("sum_doubles")) return rc;
+    CG_CHECK_HIP(hipMemcpyAsync(result, total, sizeof(double), hipMemcpyDeviceToHost, s));
+    CG_CHECK_HIP(hipStreamSynchronize(s));
+    return CG_OK;
+}
+
+}  // extern "C"

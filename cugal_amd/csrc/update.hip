@@ -1,0 +1,111 @@
+// Frank-Wolfe step of the quasi-permutation, cugal/pred.py:77-86 + :133-141
+// (fused form of cuda_kernels/update_quasi_permutation.cu:7-35):
+//   plan = exp(-C/reg + u_i + v_j),  gap = sum (plan - P) C,  P += a (plan - P)
+// One read of C, one read and one write of P: 12 n^2 bytes.  The gap is
+// accumulated in FP64 and reduced in a fixed order (deterministic).
+#include "common.hpp"
+#include "cugal_hip.h"
+
+namespace cg {
+namespace {
+
+constexpr int kUpdThreads = 256;
+constexpr int kUpdMaxBlocks = 2048;
+
+template <bool VEC>
+__global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, const float* __restrict__ C, int n,
+                                                        const float* __restrict__ u, const float* __restrict__ v,
+                                                        float alpha, float neg_rinv, int rows_per_blk,
+                                                        double* __restrict__ part) {
+    __shared__ double scratch[kUpdThreads / kWave];
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    double gap = 0.0;
+    for (int i = r0; i < r1; ++i) {
+        const int64_t base = static_cast<int64_t>(i) * n;
+        const float ui = u[i];
+        float rowgap = 0.0f;
+        if (VEC) {
+            for (int j = threadIdx.x * 4; j < n; j += kUpdThreads * 4) {
+                const float4 c = ld_stream4(C + base + j);
+                const float4 vv = *reinterpret_cast<const float4*>(v + j);
+                float4 p = *reinterpret_cast<const float4*>(P + base + j);
+                const float e0 = exp2_fast((fmaf(c.x, neg_rinv, ui) + vv.x) * kLog2e) - p.x;
+                const float e1 = exp2_fast((fmaf(c.y, neg_rinv, ui) + vv.y) * kLog2e) - p.y;
+                const float e2 = exp2_fast((fmaf(c.z, neg_rinv, ui) + vv.z) * kLog2e) - p.z;
+                const float e3 = exp2_fast((fmaf(c.w, neg_rinv, ui) + vv.w) * kLog2e) - p.w;
+                rowgap += (e0 * c.x + e1 * c.y) + (e2 * c.z + e3 * c.w);
+                p.x = fmaf(alpha, e0, p.x);
+                p.y = fmaf(alpha, e1, p.y);
+                p.z = fmaf(alpha, e2, p.z);
+                p.w = fmaf(alpha, e3, p.w);
+                *reinterpret_cast<float4*>(P + base + j) = p;
+            }
+        } else {
+            for (int j = threadIdx.x; j < n; j += kUpdThreads) {
+                const float c = C[base + j];
+                const float p = P[base + j];
+                const float d = exp2_fast((fmaf(c, neg_rinv, ui) + v[j]) * kLog2e) - p;
+                rowgap += d * c;
+                P[base + j] = fmaf(alpha, d, p);
+            }
+        }
+        gap += static_cast<double>(rowgap);
+    }
+    gap = block_sum_double(gap, scratch);
+    if (threadIdx.x == 0) part[blockIdx.x] = gap;
+}
+
+__global__ void sum_parts(const double* __restrict__ in, int n, double* __restrict__ out) {
+    __shared__ double scratch[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += in[i];
+    s = block_sum_double(s, scratch);
+    if (threadIdx.x == 0) *out = s;
+}
+
+int update_blocks(int64_t n, int* rows_per_blk) {
+    const int64_t rpb = std::max<int64_t>(1, div_up(n, kUpdMaxBlocks));
+    *rows_per_blk = static_cast<int>(rpb);
+    return static_cast<int>(div_up(n, rpb));
+}
+
+}  // namespace
+}  // namespace cg
+
+using namespace cg;
+
+extern "C" {
+
+size_t cg_update_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    int rpb = 0;
+    return sizeof(double) * static_cast<size_t>(update_blocks(n, &rpb)) + 256;
+}
+
+int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
+                                    float alpha, float regularization, double* gap_device, void* workspace,
+                                    size_t workspace_bytes, cg_stream_t stream) {
+    if (!P || !C || !u || !v || !gap_device || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_update_quasi_permutation_log: bad arguments");
+    if (!(regularization > 0.0f)) return fail(CG_EINVAL, "regularization must be > 0");
+    if (!workspace || workspace_bytes < cg_update_workspace(n))
+        return fail(CG_EWORKSPACE, "cg_update_quasi_permutation_log: workspace too small");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int rpb = 0;
+    const int blocks = update_blocks(n, &rpb);
+    double* part = static_cast<double*>(workspace);
+    const bool vec = (n % 4 == 0) && aligned16(P) && aligned16(C) && aligned16(v);
+    const float neg_rinv = -1.0f / regularization;
+    if (vec)
+        hipLaunchKernelGGL(fw_update<true>, dim3(blocks), dim3(kUpdThreads), 0, s, P, C, static_cast<int>(n), u, v,
+                           alpha, neg_rinv, rpb, part);
+    else
+        hipLaunchKernelGGL(fw_update<false>, dim3(blocks), dim3(kUpdThreads), 0, s, P, C, static_cast<int>(n), u, v,
+                           alpha, neg_rinv, rpb, part);
+    if (int rc = check_launch("fw_update")) return rc;
+    hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, part, blocks, gap_device);
+    return check_launch("sum_parts");
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+/*
+ * cugal_hip.h - C ABI of libcugal_hip.so, the MI355X (gfx950) kernels of the
+ * FUGAL graph-alignment pipeline.
+ *
+ * Conventions (all entry points):
+ *   - plain device pointers and sizes; no framework types;
+ *   - matrices are n x n FP32, row-major, contiguous (leading dimension n);
+ *   - CSR adjacency: int32 col_indices[nnz], int32 row_pointers[n + 1], the
+ *     columns of a row sorted ascending;
+ *   - node-feature matrices are n x 4 FP32 row-major
+ *     (degree, clustering, mean neighbour degree, mean neighbour clustering);
+ *   - `stream` is a hipStream_t (NULL = the null stream); work is enqueued on
+ *     it and nothing synchronises the device unless the entry point says so;
+ *   - temporaries come from a caller-provided device workspace whose size the
+ *     matching *_workspace() function returns; the library never allocates
+ *     device memory;
+ *   - the return value is 0 on success or a negative code, with a message
+ *     available from cg_last_error() (thread-local).
+ *
+ * Each declaration names the reference interface it replaces
+ * (SebastianRueClausen/cuGAL, path:line).
+ */
+#ifndef CUGAL_HIP_H
+#define CUGAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* cg_stream_t;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cg_last_error(void);
+/* ABI version of the library (major * 100 + minor). */
+int cg_version(void);
+
+/* ------------------------------------------------------------------------
+ * Adjacency.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces create_adjacency (cuda_kernels/adjacency.cu:228-259, bound at
+ * cuda_kernels/kernels.cpp:38-40) and the CSR part of Adjacency.from_graph
+ * (cugal/adjacency.py:46-69).  `edges` holds edge_count int32 pairs (x, y);
+ * each pair contributes the entry A[y][x] and, when `symmetrize` is set, also
+ * A[x][y] (the reference appends the reversed list in Python,
+ * cugal/adjacency.py:53-55).  Writes row_pointers[node_count + 1] and
+ * col_indices[nnz] with nnz = edge_count * (symmetrize ? 2 : 1); columns are
+ * sorted within each row, and rows without entries (including leading and
+ * trailing isolated vertices) get empty ranges. */
+size_t cg_csr_build_workspace(int64_t edge_count, int64_t node_count, int symmetrize);
+int cg_csr_build(const int32_t* edges, int64_t edge_count, int64_t node_count, int symmetrize,
+                 int32_t* col_indices, int32_t* row_pointers, void* workspace, size_t workspace_bytes,
+                 cg_stream_t stream);
+
+/* Replaces adjacency_matmul (cuda_kernels/adjacency.cu:77-150, bound at
+ * cuda_kernels/kernels.cpp:34-37; used by Adjacency.mul,
+ * cugal/adjacency.py:128-143): out = (negate ? -A : A) @ matrix for a
+ * contiguous n x n matrix. */
+int cg_adjacency_matmul(const int32_t* col_indices, const int32_t* row_pointers, int64_t n,
+                        const float* matrix, float* out, int negate, cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Node features.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces graph_features (cuda_kernels/feature_extraction.cu:67-84, bound
+ * at cuda_kernels/kernels.cpp:43): per vertex the degree and the clustering
+ * coefficient t / (d (d - 1)) where t = sum over neighbours u of
+ * |N(v) & N(u)| (twice the triangle count).  `triangles` (nullable)
+ * receives t as an exact int32. */
+int cg_graph_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, float* clustering,
+                      float* degrees, int32_t* triangles, cg_stream_t stream);
+
+/* Replaces average_neighbor_features (cuda_kernels/feature_extraction.cu:
+ * 108-125, bound at cuda_kernels/kernels.cpp:44-47): averages[v] = mean of
+ * features over N(v), 0 for isolated vertices. */
+int cg_average_neighbor_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n,
+                                 const float* features, float* averages, cg_stream_t stream);
+
+/* Replaces extract_features_cuda + the mu scaling of Features.create
+ * (cugal/feature_extraction.py:15-38, :91-92) in one call: out (n x 4) =
+ * mu * (degree, clustering, mean neighbour degree, mean neighbour
+ * clustering).  `triangles` (nullable) as in cg_graph_features.  Integer
+ * sums are exact; the divisions are correctly rounded FP32. */
+size_t cg_node_features_workspace(int64_t n);
+int cg_node_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, float mu, float* out,
+                     int32_t* triangles, void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
+/* Ego-net edge count per vertex, deg(v) + t(v) / 2: the feature of
+ * FUGAL/pred.py:68-72 (nx.ego_graph(G, v).number_of_edges() on a simple
+ * graph).  Needs `triangles` from cg_graph_features / cg_node_features. */
+int cg_egonet_edges(const int32_t* row_pointers, const int32_t* triangles, int64_t n, int32_t* out,
+                    cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Cost / gradient.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces add_distance (cuda_kernels/distance.cu:35-53, bound at
+ * cuda_kernels/kernels.cpp:60): out[x][y] += ||source[x] - target[y]||_2
+ * over the 4 features. */
+int cg_add_distance(const float* source, const float* target, int64_t n, float* out, cg_stream_t stream);
+
+/* Replaces regularize (cuda_kernels/regularize.cu:24-39, bound at
+ * cuda_kernels/kernels.cpp:63): gradient += iteration - 2 * iteration * P,
+ * 64-bit indexing. */
+int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_stream_t stream);
+
+/* The whole Frank-Wolfe gradient of cugal/pred.py:38-74 in two kernels for
+ * undirected graphs (A, B symmetric, given as CSR):
+ *   gradient = -2 A P B + D + regularizer_factor * lambda * (1 - 2 P)
+ * where D[x][y] = ||source_features[x] - target_features[y]||_2 is recomputed
+ * on the fly (no n x n distance matrix; features may be NULL for D = 0).
+ * regularizer_factor = 1 is sparse_gradient (cugal/pred.py:54-74); 2 is
+ * dense_gradient, which adds lambda (1 - 2P) twice (cugal/pred.py:46-50).
+ * Workspace: one n x n FP32 intermediate (P B). */
+size_t cg_gradient_workspace(int64_t n);
+int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
+                int64_t n, const float* P, const float* source_features, const float* target_features,
+                int lambda_, float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Sinkhorn (log domain, FP32).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces sinkhorn_log_step (cuda_kernels/sinkhorn_log.cu:100-114, bound at
+ * cuda_kernels/kernels.cpp:50-53): for each row r of the n x n matrix K,
+ *   out[r] = (1 - momentum) out[r] - momentum * LSE_c(K[r][c] + add[c]). */
+int cg_sinkhorn_log_step(const float* K, int64_t n, const float* add, float* out, float momentum,
+                         cg_stream_t stream);
+
+/* Replaces sinkhorn_log_marginal (cuda_kernels/sinkhorn_log.cu:88-98, bound
+ * at cuda_kernels/kernels.cpp:54-57): sum_r |sum_c exp(K[r][c] + v[c] + u[r])
+ * - 1| accumulated in FP64; synchronises the stream and writes *result on the
+ * host. */
+size_t cg_sinkhorn_log_marginal_workspace(int64_t n);
+int cg_sinkhorn_log_marginal(const float* K, int64_t n, const float* u, const float* v, double* result,
+                             void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
+/* Replaces SinkhornState.solve_log (cugal/sinkhorn.py:90-120) including its
+ * per-iteration kernel launches.  C is the cost (the FW gradient); K = -C /
+ * regularization is never materialised.  u, v (device, n floats, natural-log
+ * potentials) are the warm start and receive the result.  Per iteration
+ * v = -LSE_i(K + u), u = -LSE_j(K + v); every eval_freq iterations (not at
+ * iteration 0) the column-marginal L1 error is recorded (FP64) and the solve
+ * stops when it is below `threshold`.  momentum_start < 0 disables Lehmann
+ * momentum (cugal/sinkhorn.py:78-88); otherwise it is the reference's
+ * sinkhorn_momentum_start.  fused = 1 reads C once per iteration (n <=
+ * 32768, n % 4 == 0, 16-byte aligned C; otherwise the two-sweep structure is
+ * used, same results); fused = 0 reads it twice (column then row sweep).
+ * Synchronises the stream once at the end.  Host outputs: errors[0 ..
+ * min(count, errors_capacity)), *error_count, *iterations (the reference's
+ * SinkhornProfile.iteration_count), and when non-NULL the summed duration and
+ * number of the C-reading sweep kernels (for roofline reporting; adds event
+ * records between launches). */
+size_t cg_sinkhorn_workspace(int64_t n, int fused, int max_iterations, int eval_freq);
+int cg_sinkhorn_fused_supported(int64_t n);
+int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float* u, float* v, int max_iterations,
+                          double threshold, int eval_freq, int momentum_start, int fused, double* errors,
+                          int errors_capacity, int* error_count, int* iterations, float* sweep_ms,
+                          int* sweep_launches, void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Frank-Wolfe update.
+ * ---------------------------------------------------------------------- */
+
+/* Replaces update_quasi_permutation_log (cuda_kernels/update_quasi_permutation
+ * .cu:40-56, bound at cuda_kernels/kernels.cpp:66; called at
+ * cugal/pred.py:134-136): with plan = exp(-C/reg + u_i + v_j),
+ *   gap = sum (plan - P) * C  (FP64),   P += alpha * (plan - P).
+ * *gap_device (a device double) receives the signed sum; the reference
+ * reports |gap|.  Takes the cost C instead of K = -C/reg. */
+size_t cg_update_workspace(int64_t n);
+int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
+                                    float alpha, float regularization, double* gap_device, void* workspace,
+                                    size_t workspace_bytes, cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Rounding to a permutation (LAP).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces greedy_lap (cugal/greedy_lap.py:7-63) for
+ *   method 0 = GREEDY (rows in index order, :10-19),
+ *   method 1 = DOUBLE_GREEDY (rows by descending row maximum, :50-63; ties
+ *              in row order, i.e. a stable sort).
+ * For each row in order, the columns already taken are multiplied by 0 in P
+ * (in place, as the reference does) and the first index of the row maximum
+ * is taken.  assignment (device, n int32) receives the column of each row. */
+size_t cg_lap_workspace(int64_t n);
+int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* workspace, size_t workspace_bytes,
+                  cg_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUGAL_HIP_H */
