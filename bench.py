@@ -1,0 +1,206 @@
+"""Benchmark of the FUGAL alignment path on one MI355X.
+
+Workload (default): Watts-Strogatz n=32768, k=20, rewire p=0.1, 1% edge noise
+(remove + refill) and a seeded permutation; cuGAL's published GPU settings
+(LOG Sinkhorn FP32, regularization 0.3, threshold 1e-3, <= 500 iterations,
+mu 2, 15 lambda stages x 10 Frank-Wolfe steps, sparse adjacency) with the
+DOUBLE_GREEDY LAP.  One step = one whole alignment (features, gradient,
+Sinkhorn, update, LAP, result copied to the host), inputs resident on the
+device before the timed region.
+
+Reported: `value` = seconds per alignment on the fused path ("mixed": the
+cost is read once per Sinkhorn iteration), beside the plain two-sweep path
+on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
+4 n^2 per sweep over the HIP-event time of the sweep launches) and the FP64
+CPU oracle timed on a bounded sample of the same Sinkhorn work.
+"""
+
+from __future__ import annotations
+
+import argparse
+import copy
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_config(fused: bool, args):
+    import torch
+
+    from cugal_amd import Config, HungarianMethod, SinkhornMethod
+
+    return Config(
+        device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+        sinkhorn_regularization=args.reg, sinkhorn_iterations=args.sinkhorn_iterations,
+        sinkhorn_threshold=args.threshold, sinkhorn_eval_freq=10, mu=args.mu, iter_count=args.iter_count,
+        frank_wolfe_iter_count=args.fw_iter_count, use_sparse_adjacency=True,
+        hungarian_method=HungarianMethod.DOUBLE_GREEDY, sinkhorn_fused=fused,
+    )
+
+
+def run_alignment(inst, config, timing: bool):
+    import torch
+
+    from cugal_amd.pred import cugal
+    from cugal_amd.profile import Profile
+    from cugal_amd.sinkhorn import SweepTiming
+    from cugal_amd.synthetic import alignment_accuracy, answer_columns
+
+    source, target = copy.deepcopy(inst.source), copy.deepcopy(inst.target)
+    SweepTiming.reset(timing)
+    profile = Profile()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, mapping = cugal(source, target, config, profile)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    sweeps = (SweepTiming.total_ms, SweepTiming.launches)
+    SweepTiming.reset(False)
+    return elapsed, acc, profile, sweeps
+
+
+def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: int):
+    """FP64 oracle log-Sinkhorn (torch.logsumexp, cugal/sinkhorn.py:107-111)
+    on a cost of the bench's size; times whole iterations until the budget
+    is spent and scales to the GPU run's Sinkhorn iteration count."""
+    import torch
+
+    from oracle import fugal as oracle
+
+    g = torch.Generator().manual_seed(0)
+    C = torch.rand((n, n), generator=g, dtype=torch.float64) * 4.0
+    state = oracle.SinkhornState(n, "LOG")
+    done, t0 = 0, time.perf_counter()
+    while True:
+        state.solve_log(C, reg, 1, 0.0, 10)
+        done += 1
+        if time.perf_counter() - t0 > seconds_budget:
+            break
+    per_iter = (time.perf_counter() - t0) / done
+    return {
+        "value": per_iter * total_iterations,
+        "unit": "s (Sinkhorn only, extrapolated)",
+        "cores": torch.get_num_threads(),
+        "kind": "port",
+        "sample": f"FP64 oracle log-Sinkhorn at n={n}: {done} iteration(s) timed ({per_iter:.2f} s each), "
+                  f"scaled to the {total_iterations} iterations of the GPU alignment",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--k", type=int, default=20)
+    ap.add_argument("--p", type=float, default=0.1)
+    ap.add_argument("--noise", type=float, default=0.01)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--reg", type=float, default=0.3)
+    ap.add_argument("--mu", type=float, default=2.0)
+    ap.add_argument("--threshold", type=float, default=1e-3)
+    ap.add_argument("--sinkhorn-iterations", type=int, default=500)
+    ap.add_argument("--iter-count", type=int, default=15)
+    ap.add_argument("--fw-iter-count", type=int, default=10)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain two-sweep comparison run")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        log("only --gpus 1 is supported: the alignment runs on a single MI355X")
+        return 2
+
+    import torch
+
+    from cugal_amd.synthetic import watts_strogatz
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm device"
+    log(f"device {torch.cuda.get_device_name(0)}; generating WS(n={args.n}, k={args.k}, p={args.p})")
+    t = time.perf_counter()
+    inst = watts_strogatz(args.n, args.k, args.p, args.noise, seed=args.seed)
+    log(f"instance ready in {time.perf_counter() - t:.1f}s "
+        f"({inst.source.number_of_edges()} / {inst.target.number_of_edges()} edges)")
+
+    # Kernel load / first-launch costs, outside every timed region.
+    small = watts_strogatz(512, 10, 0.1, 0.01, seed=1)
+    for fused in (True, False):
+        run_alignment(small, make_config(fused, argparse.Namespace(**{**vars(args), "iter_count": 1,
+                                                                      "fw_iter_count": 2})), False)
+    for w in range(args.warmup):
+        e, a, _, _ = run_alignment(inst, make_config(True, args), False)
+        log(f"warmup {w}: {e:.2f}s acc {a:.4f}")
+
+    times, accs, sweep_ms, sweep_n, iters_total, profile = [], [], 0.0, 0, 0, None
+    for s in range(args.steps):
+        e, a, profile, (ms, cnt) = run_alignment(inst, make_config(True, args), True)
+        times.append(e)
+        accs.append(a)
+        sweep_ms += ms
+        sweep_n += cnt
+        iters_total = sum(p.iteration_count for p in profile.sinkhorn_profiles)
+        log(f"step {s}: fused {e:.2f}s acc {a:.4f} sinkhorn iterations {iters_total}")
+    value = sum(times) / len(times)
+
+    plain = None
+    if not args.no_plain:
+        e, a, pprof, (pms, pcnt) = run_alignment(inst, make_config(False, args), True)
+        plain = {"seconds": e, "accuracy": a,
+                 "sinkhorn_iterations": sum(p.iteration_count for p in pprof.sinkhorn_profiles),
+                 "sweep_ms": pms, "sweep_launches": pcnt}
+        log(f"plain two-sweep: {e:.2f}s acc {a:.4f}")
+
+    n = args.n
+    avg_sweep_ms = sweep_ms / max(1, sweep_n)
+    sweep_bytes = 4.0 * n * n
+    achieved = sweep_bytes / (avg_sweep_ms * 1e-3) / 1e9 if sweep_n else None
+    roofline = {
+        "kernel": "sk_fused_sweep (one read of C per Sinkhorn iteration)",
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+        "bytes_per_launch": sweep_bytes, "avg_launch_ms": avg_sweep_ms, "launches": sweep_n,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the FP64 CPU oracle sample")
+        cpu = cpu_baseline(min(n, 16384), args.reg, args.cpu_seconds, iters_total)
+        if n > 16384:
+            scale = (n / 16384) ** 2
+            cpu["value"] *= scale
+            cpu["sample"] += f"; measured at n=16384 and scaled by (n/16384)^2 = {scale:.1f}"
+
+    phase = {k.value: v for k, v in profile.phase_times.items()} if profile else {}
+    out = {
+        "metric": "align time (s) n=32k mixed vs FP32 plain @ equal node acc; Sinkhorn HBM GB/s",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"watts_strogatz n={n} k={args.k} p={args.p}, {args.noise:.0%} edge noise, "
+                               f"seeded permutation; LOG reg {args.reg}, thr {args.threshold}, mu {args.mu}, "
+                               f"{args.iter_count}x{args.fw_iter_count} FW, DOUBLE_GREEDY",
+                   "n": n, "parallelism": "single GPU"},
+        "accuracy": sum(accs) / len(accs), "sinkhorn_iterations": iters_total,
+        "sinkhorn_gbs": achieved, "phase_times": phase, "plain": plain,
+        "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,241 @@
+"""The FUGAL alignment pipeline, the surface of cugal/pred.py:30-249.
+
+``cugal(source, target, config, profile)`` pads the two graphs to one node
+range, builds their CSR adjacency and structural features on the device,
+runs ``iter_count`` lambda stages of ``frank_wolfe_iter_count`` Frank-Wolfe
+steps (gradient -> log-Sinkhorn -> quasi-permutation update) and rounds the
+result with the configured LAP.  Every n x n pass runs in the HIP kernels of
+libcugal_hip.so; the host only sequences them.  Supported: config.device a
+ROCm device ("cuda"), dtype torch.float32, SinkhornMethod.LOG, undirected
+graphs.  Anything else raises rather than falling back.
+"""
+
+from __future__ import annotations
+
+import networkx as nx
+import numpy as np
+import scipy.optimize
+import scipy.sparse
+import scipy.sparse.csgraph
+import torch
+
+from . import _native as nat
+from .adjacency import Adjacency
+from .config import Config, HungarianMethod, SinkhornMethod
+from .feature_extraction import Features
+from .greedy_lap import greedy_lap
+from .profile import Phase, Profile, SinkhornProfile, TimeStamp
+from .sinkhorn import LogKernel, SinkhornState
+
+
+def _check_config(config: Config) -> None:
+    if "cuda" not in config.device:
+        raise NotImplementedError(
+            "this library runs the alignment on a ROCm device: set Config(device='cuda', ...). "
+            "The CPU path of the reference is not provided."
+        )
+    if config.dtype != torch.float32:
+        raise NotImplementedError("the HIP kernels compute in FP32: set Config(dtype=torch.float32)")
+    if config.sinkhorn_method != SinkhornMethod.LOG:
+        raise NotImplementedError("the HIP Sinkhorn is the log-domain one: set sinkhorn_method=SinkhornMethod.LOG")
+    if config.gradient_precision != "fp32":
+        raise NotImplementedError(f"gradient_precision {config.gradient_precision!r} is not available")
+
+
+def _gradient_into(
+    out: torch.Tensor, A: Adjacency, B: Adjacency, P: torch.Tensor, features, iteration: int, factor: float
+) -> torch.Tensor:
+    """out = -2 A P B + D + factor * iteration * (1 - 2P) for symmetric A, B.
+    `features` is a Features (distance recomputed in the kernel), an n x n
+    distance tensor (added afterwards) or None."""
+    n = P.shape[0]
+    dev = P.device
+    fs = ft = None
+    if isinstance(features, Features):
+        fs, ft = features.source.contiguous(), features.target.contiguous()
+    ws = nat.workspace(dev, nat.query("cg_gradient_workspace", n), "gradient")
+    nat.call(
+        "cg_gradient", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(B.col_indices),
+        nat.ptr(B.row_pointers), n, nat.ptr(P), nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
+        nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    if isinstance(features, torch.Tensor):
+        out += features
+    return out
+
+
+def add_feature_distance(gradient: torch.Tensor, features: torch.Tensor | Features) -> torch.Tensor:
+    """gradient += D (cugal/pred.py:30-35)."""
+    if isinstance(features, Features):
+        return features.add_distance(gradient)
+    gradient += features
+    return gradient
+
+
+def _is_symmetric(M: torch.Tensor) -> bool:
+    return bool(torch.equal(M, M.T))
+
+
+def dense_gradient(A: torch.Tensor, B: torch.Tensor, P: torch.Tensor, features, iteration: int) -> torch.Tensor:
+    """-A^T P B - A P B^T + D + 2 iteration (1 - 2P) for dense adjacency
+    tensors (cugal/pred.py:38-51, whose regulariser is added twice)."""
+    if not (_is_symmetric(A) and _is_symmetric(B)):
+        raise NotImplementedError("the HIP gradient handles undirected graphs (symmetric adjacency)")
+    out = torch.empty_like(P)
+    return _gradient_into(out, Adjacency.from_dense(A), Adjacency.from_dense(B), P, features, iteration, 2.0)
+
+
+def sparse_gradient(A: Adjacency, B: Adjacency, A_transpose: Adjacency, B_transpose: Adjacency, P: torch.Tensor,
+                    features, iteration: int) -> torch.Tensor:
+    """-2 A P B + D + iteration (1 - 2P) for undirected graphs
+    (cugal/pred.py:54-74)."""
+    if not (A is A_transpose and B is B_transpose):
+        raise NotImplementedError("the HIP gradient handles undirected graphs (A is A_transpose)")
+    out = torch.empty_like(P)
+    return _gradient_into(out, A, B, P, features, iteration, 1.0)
+
+
+def update_quasi_permutation_log(P: torch.Tensor, K: LogKernel, u: torch.Tensor, v: torch.Tensor, alpha: float,
+                                 sinkhorn_regularization: float | None = None, gap: torch.Tensor | None = None):
+    """P += alpha (exp(K + u + v) - P) in one pass; returns |sum C (plan - P)|
+    as a float, or leaves the signed sum in `gap` (device FP64) without
+    synchronising when `gap` is given (cuda_kernels/update_quasi_permutation.cu:40-56)."""
+    if not isinstance(K, LogKernel):
+        K = LogKernel(-K * sinkhorn_regularization, sinkhorn_regularization)
+    n = P.shape[0]
+    dev = P.device
+    out = torch.empty(1, dtype=torch.float64, device=dev) if gap is None else gap
+    ws = nat.workspace(dev, nat.query("cg_update_workspace", n), "update")
+    nat.call(
+        "cg_update_quasi_permutation_log", nat.ptr(P), nat.ptr(K.cost), n, nat.ptr(u), nat.ptr(v), float(alpha),
+        float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    return abs(float(out.item())) if gap is None else None
+
+
+def update_quasi_permutation(P: torch.Tensor, K, u: torch.Tensor, v: torch.Tensor, alpha: float,
+                             sinkhorn_method: SinkhornMethod) -> torch.Tensor:
+    """Reference-compatible form (cugal/pred.py:77-86): updates P and returns
+    the step alpha (plan - P)."""
+    if sinkhorn_method != SinkhornMethod.LOG:
+        raise NotImplementedError("only the log-domain update is implemented")
+    before = P.clone()
+    update_quasi_permutation_log(P, K, u, v, alpha, gap=torch.empty(1, dtype=torch.float64, device=P.device))
+    return P - before
+
+
+def _adjacency_of(graph, config: Config) -> Adjacency:
+    if isinstance(graph, Adjacency):
+        return graph
+    if isinstance(graph, torch.Tensor):
+        if not _is_symmetric(graph):
+            raise NotImplementedError("the HIP gradient handles undirected graphs (symmetric adjacency)")
+        return Adjacency.from_dense(graph.to(config.device))
+    if nx.is_directed(graph):
+        raise NotImplementedError("the HIP gradient handles undirected graphs")
+    return Adjacency.from_graph(graph, config.device)
+
+
+def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profile) -> torch.Tensor:
+    """The Frank-Wolfe loop of cugal/pred.py:89-148: lambda = 0..iter_count-1,
+    step it = 1..frank_wolfe_iter_count with alpha = 2 / (2 + it), Sinkhorn
+    duals warm-started across all steps."""
+    _check_config(config)
+    A = _adjacency_of(A, config)
+    B = _adjacency_of(B, config)
+    n = A.number_of_nodes()
+    if B.number_of_nodes() != n:
+        raise ValueError(f"graphs differ in size ({n} vs {B.number_of_nodes()})")
+    dev = A.row_pointers.device
+    sinkhorn_state = SinkhornState(n, config)
+    P = torch.full((n, n), 1.0 / n, device=dev, dtype=torch.float32)
+    gradient = torch.empty((n, n), device=dev, dtype=torch.float32)
+    gap = torch.empty(1, dtype=torch.float64, device=dev)
+    factor = 1.0 if config.use_sparse_adjacency else 2.0
+    for lam in range(config.iter_count):
+        for it in range(1, config.frank_wolfe_iter_count + 1):
+            start_time = TimeStamp(config.device)
+            _gradient_into(gradient, A, B, P, features, lam, factor)
+            profile.log_time(start_time, Phase.GRADIENT)
+
+            start_time = TimeStamp(config.device)
+            sinkhorn_profile = SinkhornProfile()
+            K, u, v = sinkhorn_state.solve(gradient, config, sinkhorn_profile)
+            profile.sinkhorn_profiles.append(sinkhorn_profile)
+            profile.log_time(start_time, Phase.SINKHORN)
+
+            alpha = 2.0 / float(2.0 + it)
+            update_quasi_permutation_log(P, K, u, v, alpha, gap=gap)
+
+            profile.frank_wolfe_iterations += 1
+            if config.frank_wolfe_threshold is not None and abs(float(gap.item())) < config.frank_wolfe_threshold:
+                break
+    return P
+
+
+def hungarian(quasi_permutation: torch.Tensor, config: Config, profile: Profile):
+    """Round to a permutation (cugal/pred.py:151-178); returns the column
+    of each row."""
+    profile.sparsity = (
+        torch.sum(torch.le(quasi_permutation, torch.finfo(quasi_permutation.dtype).eps)).item()
+        / quasi_permutation.numel()
+    )
+    start_time = TimeStamp(config.device)
+    match config.hungarian_method:
+        case HungarianMethod.SCIPY:
+            _, column_indices = scipy.optimize.linear_sum_assignment(quasi_permutation.cpu().numpy(), maximize=True)
+        case HungarianMethod.GREEDY | HungarianMethod.DOUBLE_GREEDY:
+            column_indices = greedy_lap(quasi_permutation, config)
+        case HungarianMethod.SPARSE:
+            sparse = scipy.sparse.csr_matrix(quasi_permutation.cpu().numpy())
+            _, column_indices = scipy.sparse.csgraph.min_weight_full_bipartite_matching(sparse, maximize=True)
+        case _:
+            raise NotImplementedError(f"{config.hungarian_method} is not implemented on the ROCm path")
+    profile.log_time(start_time, Phase.HUNGARIAN)
+    return column_indices
+
+
+def convert_to_permutation_matrix(quasi_permutation: torch.Tensor, config: Config, profile: Profile):
+    """(quasi-permutation as numpy, [(i, col_i)]) (cugal/pred.py:181-193).
+    The reference also builds a dense permutation matrix it never returns;
+    that allocation is skipped."""
+    col_ind = hungarian(quasi_permutation, config, profile)
+    mapping = [(i, int(col_ind[i])) for i in range(len(quasi_permutation))]
+    return quasi_permutation.cpu().numpy(), mapping
+
+
+def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile = Profile()):
+    """Run the alignment (cugal/pred.py:196-249).  Returns the quasi-
+    permutation (after rounding, as numpy) and the mapping from source to
+    target nodes.  Like the reference, pads both graphs in place with the
+    nodes of range(max label) they lack, and `profile` defaults to one shared
+    instance."""
+    _check_config(config)
+    before = TimeStamp("cpu")
+
+    node_count = max(max(source.nodes()), max(target.nodes()))
+    source.add_nodes_from(set(range(node_count)) - set(source.nodes()))
+    target.add_nodes_from(set(range(node_count)) - set(target.nodes()))
+
+    source_adj = _adjacency_of(source, config)
+    target_adj = _adjacency_of(target, config)
+
+    start_time = TimeStamp(config.device)
+    features = Features.create(source_adj, target_adj, config)
+    if config.safe_mode:
+        assert features.source.isfinite().all(), "source feature tensor has NaN values"
+        assert features.target.isfinite().all(), "target feature tensor has NaN values"
+    profile.log_time(start_time, Phase.FEATURE_EXTRACTION)
+
+    quasi_permutation = find_quasi_permutation_matrix(source_adj, target_adj, features, config, profile)
+    if config.safe_mode:
+        assert quasi_permutation.isfinite().all(), "quasi_permutation tensor has NaN values"
+
+    output = convert_to_permutation_matrix(quasi_permutation, config, profile)
+    if config.safe_mode:
+        assert np.isfinite(output[0]).all(), "output permutation matrix has NaN values"
+
+    profile.time = TimeStamp("cpu").elapsed_seconds(before)
+    profile.max_memory = torch.cuda.max_memory_allocated(config.device)
+    torch.cuda.reset_peak_memory_stats(config.device)
+    return output

@@ -1,0 +1,352 @@
+"""FP64 CPU restatement of cuGAL's alignment path (test infrastructure).
+
+Each function cites the reference lines it restates and names its pin (the
+independent check in tests/test_oracle.py).  The reference runs its CPU path
+with torch in FP64; so does this module, with the same op order where the
+reference's rounding is part of the result.
+
+Reference quirks (see the `quirks` arguments):
+  * dense_gradient adds lambda (1 - 2P) twice (cugal/pred.py:46-50): kept,
+    since it is the reference's result for use_sparse_adjacency=False;
+  * the LOG solver's marginal check overwrites K with exp(K+u+v)-u-v
+    (cugal/sinkhorn.py:67-74): off by default (quirk_log_marginal=True
+    reproduces it);
+  * DOUBLE_GREEDY's argsort is not stable (cugal/greedy_lap.py:52): a stable
+    sort is used, identical whenever the row maxima are distinct.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import networkx as nx
+import numpy as np
+import scipy.optimize
+import torch
+
+M_EPS = 1e-16
+
+
+# ---------------------------------------------------------------------------
+# Features
+# ---------------------------------------------------------------------------
+
+def extract_features_nx(G: nx.Graph) -> np.ndarray:
+    """[degree, clustering, mean neighbour degree, mean neighbour clustering]
+    per node in sorted node order, cugal/feature_extraction.py:41-64.
+    Pin: closed forms on K_n, C_n, stars, the Petersen graph; clustering
+    against brute-force triangle enumeration; neighbour means against
+    numpy adjacency products."""
+    node_list = sorted(G.nodes())
+    node_degree_dict = dict(G.degree())
+    node_clustering_dict = dict(nx.clustering(G))
+    degs = [node_degree_dict[n] for n in node_list]
+    clusts = [node_clustering_dict[n] for n in node_list]
+    neighbor_degs = [
+        np.mean([node_degree_dict[m] for m in G.neighbors(n) if m != n]) if node_degree_dict[n] > 0 else 0
+        for n in node_list
+    ]
+    neighbor_clusts = [
+        np.mean([node_clustering_dict[m] for m in G.neighbors(n) if m != n]) if node_degree_dict[n] > 0 else 0
+        for n in node_list
+    ]
+    return np.nan_to_num(np.stack((degs, clusts, neighbor_degs, neighbor_clusts)).T)
+
+
+def triangle_counts(adjacency: np.ndarray) -> np.ndarray:
+    """t(v) = sum_{u in N(v)} |N(v) & N(u)| = 2 * triangles through v =
+    (A^3)_vv, the quantity of cuda_kernels/feature_extraction.cu:41-57.
+    Pin: brute-force enumeration of vertex triples; nx.triangles."""
+    A = (adjacency != 0).astype(np.int64)
+    return np.einsum("ij,ji->i", A @ A, A)
+
+
+def egonet_edges(G: nx.Graph) -> np.ndarray:
+    """Edges of each node's ego network (FUGAL/pred.py:68-72).  Pin:
+    deg + triangles on simple graphs."""
+    return np.array([nx.ego_graph(G, v).number_of_edges() for v in sorted(G.nodes())], dtype=np.int64)
+
+
+def distance_matrix(source: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
+    """torch.cdist of the scaled features, cugal/feature_extraction.py:106-108.
+    Pin: direct FP64 Euclidean form (abs tolerance: cdist's mm path
+    rounds differently for n > 25)."""
+    return torch.cdist(source, target)
+
+
+# ---------------------------------------------------------------------------
+# Gradient
+# ---------------------------------------------------------------------------
+
+def dense_gradient(A: torch.Tensor, B: torch.Tensor, P: torch.Tensor, D: torch.Tensor, iteration: int,
+                   double_lambda: bool = True) -> torch.Tensor:
+    """-A^T P B - A P B^T + D + lambda(1 - 2P) [+ lambda - 2 lambda P again],
+    cugal/pred.py:38-51.  Pin: closed forms at P = J/n (d_A d_B^T / n) and
+    at a permutation (common-neighbour counts by brute force)."""
+    gradient = -A.T @ P @ B - A @ P @ B.T
+    gradient = gradient + D + iteration * (1 - 2 * P)
+    if double_lambda:
+        gradient += iteration - iteration * 2 * P
+    return gradient
+
+
+def sparse_gradient(A: torch.Tensor, B: torch.Tensor, P: torch.Tensor, D: torch.Tensor, iteration: int):
+    """Symmetric sparse form -2 (B (A P)^T)^T + D + lambda(1 - 2P),
+    cugal/pred.py:54-74 (A, B symmetric).  Pin: equals dense_gradient minus
+    one regulariser term on undirected graphs."""
+    gradient = (B @ (A @ P).T).T
+    gradient *= -2
+    gradient = gradient + D
+    gradient += iteration - iteration * 2 * P
+    return gradient
+
+
+# ---------------------------------------------------------------------------
+# Sinkhorn
+# ---------------------------------------------------------------------------
+
+def scale_kernel_matrix(K, u, v):
+    """cugal/sinkhorn.py:20-23 (in place)."""
+    K *= v.reshape(1, -1)
+    K *= u.reshape(-1, 1)
+    return K
+
+
+def scale_kernel_matrix_log(K, log_u, log_v):
+    """cugal/sinkhorn.py:26-30 (in place)."""
+    K += log_u[:, None]
+    K += log_v[None, :]
+    torch.exp(K, out=K)
+    return K
+
+
+def marginal_error(K, u, v) -> float:
+    """Column-marginal L1 error with in-place scale/unscale,
+    cugal/sinkhorn.py:33-40."""
+    K *= v.reshape(1, -1)
+    K *= u.reshape(-1, 1)
+    dS = torch.sum(abs(torch.sum(K, dim=0) - 1)).item()
+    K /= v.reshape(1, -1)
+    K /= u.reshape(-1, 1)
+    return dS
+
+
+@dataclass
+class SinkhornResult:
+    errors: list = field(default_factory=list)
+    iteration_count: int = 0
+
+
+class SinkhornState:
+    """Warm-started duals and the two solvers of cugal/sinkhorn.py:51-144.
+    Pin: separable costs (plan = J/n), the 2x2 closed form, doubly
+    stochastic invariants, LOG == STANDARD fixed point, the eps -> 0 limit
+    against scipy's LAP."""
+
+    def __init__(self, n: int, method: str, dtype=torch.float64):
+        if method == "LOG":
+            vector = torch.zeros(n, dtype=dtype)
+        else:
+            vector = torch.full((n,), 1 / n, dtype=dtype)
+        self.u = vector.clone()
+        self.v = vector
+
+    def lehmann_momentum(self, momentum_start, eval_freq, errors) -> float:
+        """cugal/sinkhorn.py:78-88."""
+        if momentum_start is None or len(errors) * eval_freq < momentum_start:
+            return 1.0
+        index = momentum_start // eval_freq
+        if len(errors) < index - 1:
+            return 1.0
+        ratio = min(errors[index - 1] / errors[index - 2], 0.99)
+        if math.isnan(ratio):
+            ratio = 0.99
+        return 2.0 / (1.0 + math.sqrt(1 - ratio ** (1.0 / eval_freq)))
+
+    def marginal_error_log(self, K: torch.Tensor, quirk: bool) -> float:
+        """Column-marginal L1 error of exp(K + u + v).  With `quirk`, the
+        reference's in-place version (cugal/sinkhorn.py:67-74), which leaves K
+        = exp(K+u+v) - u - v; otherwise on a copy."""
+        if quirk:
+            K += self.u[:, None]
+            K += self.v[None, :]
+            torch.exp(K, out=K)
+            K -= self.u[:, None]
+            K -= self.v[None, :]
+            return torch.sum(abs(torch.sum(K, dim=0) - 1)).item()
+        plan = torch.exp(K + self.u[:, None] + self.v[None, :])
+        return torch.sum(abs(torch.sum(plan, dim=0) - 1)).item()
+
+    def solve_log(self, C, reg, iterations, threshold, eval_freq, momentum_start=None, quirk_log_marginal=False):
+        """cugal/sinkhorn.py:90-120 (CPU branch: torch.logsumexp)."""
+        K = -C / reg
+        errors, momentum = [], 1.0
+        iteration = -1
+        for iteration in range(iterations):
+            self.v = self.v * (1 - momentum) - momentum * torch.logsumexp(K + self.u[:, None], 0)
+            self.u = self.u * (1 - momentum) - momentum * torch.logsumexp(K + self.v[None, :], 1)
+            if iteration % eval_freq == 0 and iteration != 0:
+                errors.append(self.marginal_error_log(K, quirk_log_marginal))
+                if errors[-1] < threshold:
+                    break
+                momentum = self.lehmann_momentum(momentum_start, eval_freq, errors)
+        return K, self.u, self.v, SinkhornResult(errors, iteration + 1)
+
+    def solve_standard(self, C, reg, iterations, threshold, eval_freq, momentum_start=None):
+        """cugal/sinkhorn.py:122-144."""
+        K = torch.exp(C / -reg)
+        errors, momentum = [], 1
+        iteration = -1
+        for iteration in range(iterations):
+            self.v = self.v ** (1 - momentum) * (1 / (self.u @ K + M_EPS)) ** momentum
+            self.u = self.u ** (1 - momentum) * (1 / (K @ self.v + M_EPS)) ** momentum
+            if iteration % eval_freq == 0:
+                errors.append(marginal_error(K, self.u, self.v))
+                if errors[-1] < threshold:
+                    break
+                momentum = self.lehmann_momentum(momentum_start, eval_freq, errors)
+        return K, self.u, self.v, SinkhornResult(errors, iteration + 1)
+
+
+def plan_log(K, u, v) -> torch.Tensor:
+    return torch.exp(K + u[:, None] + v[None, :])
+
+
+def plan_standard(K, u, v) -> torch.Tensor:
+    return K * v.reshape(1, -1) * u.reshape(-1, 1)
+
+
+# ---------------------------------------------------------------------------
+# Frank-Wolfe update
+# ---------------------------------------------------------------------------
+
+def update_quasi_permutation(P, plan, alpha) -> torch.Tensor:
+    """P += alpha (plan - P); returns the step (cugal/pred.py:77-86)."""
+    q = plan.clone()
+    q -= P
+    q *= alpha
+    P += q
+    return q
+
+
+def duality_gap(gradient, step, alpha) -> float:
+    """|sum G (step / alpha)| (cugal/pred.py:140-141)."""
+    return abs(torch.sum(gradient * (step / alpha)).item())
+
+
+# ---------------------------------------------------------------------------
+# LAP
+# ---------------------------------------------------------------------------
+
+def greedy_lap(cost_matrix: torch.Tensor, method: str) -> list[int]:
+    """GREEDY (cugal/greedy_lap.py:10-19) and DOUBLE_GREEDY (:50-63), in
+    place on cost_matrix like the reference, with a stable row order.
+    Pin: brute force over permutations where greedy is optimal; hand-built
+    cases of the masking rule."""
+    n = cost_matrix.size(1)
+    taken = torch.ones(n, dtype=cost_matrix.dtype)
+    res = [0] * n
+    if method == "GREEDY":
+        order = range(cost_matrix.size(0))
+    elif method == "DOUBLE_GREEDY":
+        max_values_in_rows = cost_matrix.max(dim=1).values
+        order = torch.sort(max_values_in_rows, descending=True, stable=True).indices.tolist()
+    else:
+        raise ValueError(method)
+    for row in order:
+        cost_matrix[row] *= taken
+        m = int(cost_matrix[row].argmax())
+        taken[m] = 0
+        res[row] = m
+    return res
+
+
+def scipy_lap(quasi_permutation: torch.Tensor) -> np.ndarray:
+    """SCIPY branch, cugal/pred.py:156-158.  Pin: brute force for n <= 7."""
+    _, col = scipy.optimize.linear_sum_assignment(quasi_permutation.numpy(), maximize=True)
+    return col
+
+
+# ---------------------------------------------------------------------------
+# The whole path
+# ---------------------------------------------------------------------------
+
+@dataclass
+class OracleConfig:
+    """The reference Config fields the path reads (cugal/config.py:59-120)."""
+
+    sinkhorn_regularization: float = 1.0
+    sinkhorn_method: str = "STANDARD"
+    sinkhorn_iterations: int = 500
+    sinkhorn_threshold: float = 1e-3
+    sinkhorn_eval_freq: int = 10
+    mu: float = 0.5
+    iter_count: int = 15
+    frank_wolfe_iter_count: int = 10
+    frank_wolfe_threshold: float | None = None
+    use_sparse_adjacency: bool = False
+    sinkhorn_momentum_start: int | None = None
+    hungarian_method: str = "DOUBLE_GREEDY"
+    quirk_log_marginal: bool = False
+
+
+@dataclass
+class OracleRun:
+    P: torch.Tensor
+    mapping: list
+    sinkhorn_iterations: list
+    duality_gaps: list
+
+
+def pad_graphs(source: nx.Graph, target: nx.Graph):
+    """cugal/pred.py:209-212 (in place; range excludes the maximum label)."""
+    node_count = max(max(source.nodes()), max(target.nodes()))
+    source.add_nodes_from(set(range(node_count)) - set(source.nodes()))
+    target.add_nodes_from(set(range(node_count)) - set(target.nodes()))
+
+
+def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.float64) -> OracleRun:
+    """cugal() with the CPU path (cugal/pred.py:196-249, :89-148, :151-193).
+    Pin: exact recovery of the permutation on noise-free isomorphic pairs
+    without automorphisms."""
+    pad_graphs(source, target)
+    fs = torch.tensor(extract_features_nx(source), dtype=dtype) * config.mu
+    ft = torch.tensor(extract_features_nx(target), dtype=dtype) * config.mu
+    D = distance_matrix(fs, ft)
+    A = torch.tensor(nx.to_numpy_array(source, nodelist=sorted(source.nodes())), dtype=dtype)
+    B = torch.tensor(nx.to_numpy_array(target, nodelist=sorted(target.nodes())), dtype=dtype)
+    n = A.shape[0]
+    state = SinkhornState(n, config.sinkhorn_method, dtype)
+    P = torch.full((n, n), 1 / n, dtype=dtype)
+    iterations, gaps = [], []
+    for lam in range(config.iter_count):
+        for it in range(1, config.frank_wolfe_iter_count + 1):
+            if config.use_sparse_adjacency:
+                G = sparse_gradient(A, B, P, D, lam)
+            else:
+                G = dense_gradient(A, B, P, D, lam)
+            if config.sinkhorn_method == "LOG":
+                K, u, v, res = state.solve_log(G, config.sinkhorn_regularization, config.sinkhorn_iterations,
+                                               config.sinkhorn_threshold, config.sinkhorn_eval_freq,
+                                               config.sinkhorn_momentum_start, config.quirk_log_marginal)
+                plan = plan_log(K, u, v)
+            else:
+                K, u, v, res = state.solve_standard(G, config.sinkhorn_regularization, config.sinkhorn_iterations,
+                                                    config.sinkhorn_threshold, config.sinkhorn_eval_freq,
+                                                    config.sinkhorn_momentum_start)
+                plan = plan_standard(K, u, v)
+            iterations.append(res.iteration_count)
+            alpha = 2.0 / float(2.0 + it)
+            step = update_quasi_permutation(P, plan, alpha)
+            gap = duality_gap(G, step, alpha)
+            gaps.append(gap)
+            if config.frank_wolfe_threshold is not None and gap < config.frank_wolfe_threshold:
+                break
+    Q = P.clone()
+    if config.hungarian_method in ("GREEDY", "DOUBLE_GREEDY"):
+        col = greedy_lap(Q, config.hungarian_method)
+    else:
+        col = list(scipy_lap(Q))
+    mapping = [(i, int(col[i])) for i in range(n)]
+    return OracleRun(Q, mapping, iterations, gaps)

@@ -1,0 +1,75 @@
+"""End-to-end cugal() on the device against the FP64 oracle pipeline
+(cugal/pred.py:196-249) on seeded synthetic instances."""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from cugal_amd import Config, HungarianMethod, SinkhornMethod, cugal
+from cugal_amd.profile import Phase, Profile
+from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi, watts_strogatz
+from oracle import fugal as oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _configs(sparse, reg, mu, iters, fw, lap="DOUBLE_GREEDY", fused=True, thr=1e-3):
+    gpu = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                 sinkhorn_regularization=reg, sinkhorn_threshold=thr, mu=mu, iter_count=iters,
+                 frank_wolfe_iter_count=fw, use_sparse_adjacency=sparse, sinkhorn_fused=fused,
+                 hungarian_method=HungarianMethod[lap])
+    ref = oracle.OracleConfig(sinkhorn_regularization=reg, sinkhorn_method="LOG", sinkhorn_threshold=thr, mu=mu,
+                              iter_count=iters, frank_wolfe_iter_count=fw, use_sparse_adjacency=sparse,
+                              hungarian_method=lap)
+    return gpu, ref
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+@pytest.mark.parametrize("fused", [True, False])
+def test_pipeline_accuracy_matches_oracle(device, sparse, fused):
+    inst = erdos_renyi(400, 0.02, 0.05, seed=0)
+    gpu, ref = _configs(sparse, 0.3, 2.0, 5, 5, fused=fused)
+    profile = Profile()
+    P, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, profile)
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)
+    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    ref_acc = alignment_accuracy(inst.source_mapping, answer_columns(run.mapping))
+    assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
+    assert profile.frank_wolfe_iterations == 25
+    assert len(profile.sinkhorn_profiles) == 25
+    assert set(profile.phase_times) == {Phase.FEATURE_EXTRACTION, Phase.GRADIENT, Phase.SINKHORN, Phase.HUNGARIAN}
+    got_iters = [p.iteration_count for p in profile.sinkhorn_profiles]
+    # FP32 vs FP64 can move a threshold crossing by one check period
+    assert all(abs(a - b) <= 10 for a, b in zip(got_iters, run.sinkhorn_iterations))
+
+
+def test_isomorphic_pair_recovered(device):
+    inst = watts_strogatz(600, 10, 0.2, 0.0, seed=2)
+    gpu, _ = _configs(True, 0.3, 2.0, 6, 6)
+    _, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, Profile())
+    assert alignment_accuracy(inst.source_mapping, answer_columns(mapping)) == 1.0
+
+
+def test_fixed_iteration_quasi_permutation_matches_oracle(device):
+    # With a zero threshold every solve runs its full iteration count, so the
+    # FP32 trajectory can be compared entry by entry with the FP64 one.
+    inst = erdos_renyi(200, 0.05, 0.05, seed=4)
+    gpu, ref = _configs(True, 1.0, 1.0, 2, 3, lap="SCIPY", thr=0.0)
+    gpu = Config(**{**gpu.__dict__, "sinkhorn_iterations": 30})
+    ref.sinkhorn_iterations = 30
+    P, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, Profile())
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)
+    np.testing.assert_allclose(P, run.P.numpy(), rtol=1e-3, atol=1e-6)
+    assert [m for _, m in mapping] == [m for _, m in run.mapping]
+
+
+def test_safe_mode_and_config_errors(device):
+    inst = erdos_renyi(100, 0.05, 0.0, seed=1)
+    gpu, _ = _configs(True, 0.5, 1.0, 1, 1)
+    cfg = Config(**{**gpu.__dict__, "safe_mode": True})
+    cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), cfg, Profile())
+    with pytest.raises(NotImplementedError):
+        cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target),
+              Config(**{**gpu.__dict__, "sinkhorn_method": SinkhornMethod.MIX}), Profile())

@@ -1,0 +1,150 @@
+"""HIP log-Sinkhorn (cg_sinkhorn_log_solve) against the FP64 oracle of
+cugal/sinkhorn.py:90-120, through the C ABI."""
+
+import numpy as np
+import pytest
+import torch
+
+from cugal_amd import _native as nat
+from cugal_amd.sinkhorn import solve_log_native
+from oracle import fugal as oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cost(n, seed, scale=10.0):
+    g = torch.Generator().manual_seed(seed)
+    return torch.rand((n, n), generator=g, dtype=torch.float64) * scale
+
+
+def _run_gpu(C64, reg, iters, thr, freq, fused, device, u0=None, v0=None, momentum_start=None):
+    n = C64.shape[0]
+    C = C64.to(torch.float32).to(device).contiguous()
+    u = (torch.zeros(n) if u0 is None else u0).to(torch.float32).to(device).contiguous()
+    v = (torch.zeros(n) if v0 is None else v0).to(torch.float32).to(device).contiguous()
+    errors, count, _ = solve_log_native(C, reg, u, v, iters, thr, freq, momentum_start, fused=fused)
+    torch.cuda.synchronize()
+    return u.double().cpu(), v.double().cpu(), errors, count
+
+
+def _run_oracle(C64, reg, iters, thr, freq, u0=None, v0=None, momentum_start=None):
+    n = C64.shape[0]
+    st = oracle.SinkhornState(n, "LOG")
+    if u0 is not None:
+        st.u, st.v = u0.double().clone(), v0.double().clone()
+    K, u, v, res = st.solve_log(C64.to(torch.float32).double(), reg, iters, thr, freq, momentum_start)
+    return K, u, v, res
+
+
+# FP32 log-domain iterates against FP64: |C/reg| <= 33 here, so potentials
+# carry about 33 * 2^-24 ~ 2e-6 absolute rounding per step; 5e-5 absolute on
+# u, v and 1e-4 relative on plan entries bound the accumulated drift.
+U_ATOL = 5e-5
+PLAN_RTOL = 1e-4
+
+
+@pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
+@pytest.mark.parametrize("fused", [True, False])
+def test_fixed_iterations_match_oracle(device, n, fused):
+    C = _cost(n, seed=n)
+    reg = 0.3 if n != 333 else 1.0
+    u, v, errors, count = _run_gpu(C, reg, 40, 0.0, 10, fused, device)
+    K, ru, rv, res = _run_oracle(C, reg, 40, 0.0, 10)
+    assert count == res.iteration_count == 40
+    np.testing.assert_allclose(u.numpy(), ru.numpy(), atol=U_ATOL * max(1.0, 1.0 / reg))
+    np.testing.assert_allclose(v.numpy(), rv.numpy(), atol=U_ATOL * max(1.0, 1.0 / reg))
+    plan = oracle.plan_log(K, u, v)
+    ref_plan = oracle.plan_log(K, ru, rv)
+    np.testing.assert_allclose(plan.numpy(), ref_plan.numpy(), rtol=PLAN_RTOL * 10, atol=1e-9)
+    # errors recorded at iterations 10, 20, 30 (not 0)
+    assert len(errors) == len(res.errors) == 3
+    np.testing.assert_allclose(errors, res.errors, rtol=2e-3, atol=1e-6)
+
+
+def test_fused_equals_two_sweep(device):
+    n = 2048
+    C = _cost(n, seed=7, scale=3.0)
+    a = _run_gpu(C, 0.2, 60, 0.0, 10, True, device)
+    b = _run_gpu(C, 0.2, 60, 0.0, 10, False, device)
+    np.testing.assert_allclose(a[0].numpy(), b[0].numpy(), atol=2e-5)
+    np.testing.assert_allclose(a[1].numpy(), b[1].numpy(), atol=2e-5)
+
+
+def test_threshold_stop_and_warm_start(device):
+    n = 512
+    C1 = _cost(n, seed=11, scale=2.0)
+    C2 = C1 + 0.05 * _cost(n, seed=12, scale=1.0)
+    u, v, e1, c1 = _run_gpu(C1, 0.5, 500, 1e-3, 10, True, device)
+    K, ru, rv, r1 = _run_oracle(C1, 0.5, 500, 1e-3, 10)
+    assert c1 == r1.iteration_count
+    assert e1[-1] < 1e-3 and len(e1) == len(r1.errors)
+    # second solve warm-starts from the first one's duals
+    u2, v2, e2, c2 = _run_gpu(C2, 0.5, 500, 1e-3, 10, True, device, u0=u, v0=v)
+    st = oracle.SinkhornState(n, "LOG")
+    st.u, st.v = ru.clone(), rv.clone()
+    _, ru2, rv2, r2 = st.solve_log(C2.to(torch.float32).double(), 0.5, 500, 1e-3, 10)
+    assert c2 == r2.iteration_count
+    np.testing.assert_allclose(u2.numpy(), ru2.numpy(), atol=1e-4)
+
+
+def test_momentum_matches_oracle(device):
+    n = 300
+    C = _cost(n, seed=5, scale=4.0)
+    u, v, e, c = _run_gpu(C, 0.4, 80, 0.0, 10, True, device, momentum_start=20)
+    K, ru, rv, res = _run_oracle(C, 0.4, 80, 0.0, 10, momentum_start=20)
+    np.testing.assert_allclose(e, res.errors, rtol=5e-3, atol=1e-6)
+    np.testing.assert_allclose(u.numpy(), ru.numpy(), atol=2e-4)
+
+
+def test_separable_cost_gives_uniform_plan(device):
+    # C_ij = f_i + g_j: the scaled kernel is rank one, so one iteration
+    # reaches the plan with every entry 1/n (row and column sums 1).
+    n = 1024
+    g = torch.Generator().manual_seed(3)
+    f = torch.rand(n, generator=g, dtype=torch.float64) * 5
+    h = torch.rand(n, generator=g, dtype=torch.float64) * 5
+    C = f[:, None] + h[None, :]
+    u, v, errors, _ = _run_gpu(C, 0.7, 11, 0.0, 10, True, device)
+    plan = torch.exp(-C / 0.7 + u[:, None] + v[None, :])
+    np.testing.assert_allclose(plan.numpy(), np.full((n, n), 1.0 / n), rtol=2e-5)
+    assert errors[0] < 1e-3
+
+
+@pytest.mark.parametrize("n", [8192, 32768])
+def test_large_marginals(device, n):
+    # Size-independent properties at full size: after a solve the rows of
+    # the plan sum to 1 (the last half-step normalises rows) and the reported
+    # column error equals the column sums' L1 deviation.
+    g = torch.Generator(device=device).manual_seed(1)
+    C = torch.rand((n, n), generator=g, device=device) * 4.0
+    u = torch.zeros(n, device=device)
+    v = torch.zeros(n, device=device)
+    errors, count, _ = solve_log_native(C, 0.3, u, v, 21, 0.0, 10, None, fused=True)
+    plan = torch.exp(-C.double() / 0.3 + u.double()[:, None] + v.double()[None, :])
+    rows = plan.sum(1)
+    cols = plan.sum(0)
+    assert count == 21
+    torch.testing.assert_close(rows, torch.ones_like(rows), rtol=0, atol=5e-4)
+    col_err = float((cols - 1).abs().sum())
+    assert abs(errors[-1] - col_err) <= 1e-2 * col_err + 1e-3
+
+
+def test_reference_step_and_marginal(device):
+    # cg_sinkhorn_log_step / cg_sinkhorn_log_marginal keep the reference
+    # kernels' contracts (cuda_kernels/sinkhorn_log.cu:8-114).
+    n = 300
+    K = -_cost(n, seed=9, scale=3.0)
+    add = torch.randn(n, dtype=torch.float64)
+    out = torch.randn(n, dtype=torch.float64)
+    Kd, addd = K.float().to(device), add.float().to(device)
+    outd = out.float().to(device)
+    nat.call("cg_sinkhorn_log_step", Kd.data_ptr(), n, addd.data_ptr(), outd.data_ptr(), 0.7,
+             nat.stream_of(device))
+    ref = 0.3 * out - 0.7 * torch.logsumexp(K.float().double() + add.float().double()[None, :], 1)
+    np.testing.assert_allclose(outd.double().cpu().numpy(), ref.numpy(), atol=2e-5)
+    from cugal_amd.sinkhorn import marginal_error_log
+    u = torch.randn(n, dtype=torch.float64) * 0.1
+    v = torch.randn(n, dtype=torch.float64) * 0.1
+    got = marginal_error_log(Kd.clone(), u.float().to(device), v.float().to(device))
+    want = float((torch.exp(K.float().double() + v.float().double()[None, :] + u.float().double()[:, None]).sum(1) - 1).abs().sum())
+    assert abs(got - want) <= 1e-4 * want

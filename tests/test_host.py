@@ -1,0 +1,133 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared symbol, the Config surface matches the reference field for
+field, and the synthetic harness follows the reference's noise model."""
+
+import ast
+import os
+import re
+
+import networkx as nx
+import numpy as np
+import pytest
+import torch
+
+from cugal_amd import _native as nat
+from cugal_amd.config import Config, HungarianMethod, SinkhornMethod
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "cugal_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), f"{name} declared in cugal_hip.h but not exported"
+    assert set(declared) == set(nat.exported_symbols()), "binding and header disagree"
+    assert lib.cg_version() == 100
+
+
+def test_error_path_reports_message():
+    with pytest.raises(nat.NativeError, match="bad arguments"):
+        nat.call("cg_regularize", None, None, 0, 1, None)
+
+
+def test_workspace_queries():
+    assert nat.query("cg_sinkhorn_fused_supported", 32768) == 1
+    assert nat.query("cg_sinkhorn_fused_supported", 32769) == 0
+    fused = nat.query("cg_sinkhorn_workspace", 32768, 1, 500, 10)
+    plain = nat.query("cg_sinkhorn_workspace", 32768, 0, 500, 10)
+    # fused partials: 256 row blocks x n floats (1/128 of C); plain: pairs
+    assert fused < 0.01 * 4 * 32768 ** 2 and plain < fused
+    assert nat.query("cg_gradient_workspace", 1000) >= 4 * 1000 * 1000
+
+
+# Fields, defaults and order of cugal/config.py:55-120.
+REFERENCE_FIELDS = [
+    ("safe_mode", False), ("device", "cpu"), ("dtype", torch.float64), ("torch_sinkhorn", False),
+    ("sinkhorn_regularization", 1.0), ("sinkhorn_method", SinkhornMethod.STANDARD), ("sinkhorn_iterations", 500),
+    ("dynamic_sinkhorn_regularization", False), ("sinkhorn_scaling", 1.0), ("sinkhorn_threshold", 1e-3),
+    ("sinkhorn_eval_freq", 10), ("sinkhorn_momentum", False), ("mu", 0.5), ("iter_count", 15),
+    ("frank_wolfe_iter_count", 10), ("frank_wolfe_threshold", None), ("use_sparse_adjacency", False),
+    ("sinkhorn_cache_size", 0), ("sinkhorn_momentum_start", None), ("recompute_distance", False),
+    ("hungarian_method", HungarianMethod.DOUBLE_GREEDY),
+]
+
+
+def test_config_fields_match_reference():
+    import dataclasses
+
+    fields = dataclasses.fields(Config)
+    names = [f.name for f in fields]
+    assert names[: len(REFERENCE_FIELDS)] == [n for n, _ in REFERENCE_FIELDS]
+    default = Config()
+    for name, value in REFERENCE_FIELDS:
+        assert getattr(default, name) == value, name
+    # extension fields do not change results by default
+    assert names[len(REFERENCE_FIELDS):] == ["sinkhorn_fused", "gradient_precision"]
+
+
+def test_config_enums_match_reference():
+    assert [m.value for m in SinkhornMethod] == ["STANDARD", "LOG", "MIX"]
+    assert [m.name for m in HungarianMethod] == ["SCIPY", "GREEDY", "RAND", "MORE_RAND", "DOUBLE_GREEDY",
+                                                 "PARALLEL_GREEDY", "DENSE", "SPARSE"]
+
+
+def test_config_dict_round_trip():
+    c = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG, mu=2.0,
+               hungarian_method=HungarianMethod.SCIPY, sinkhorn_momentum_start=30)
+    d = c.to_dict()
+    assert d["dtype"] == "float32" and d["sinkhorn_method"] == "LOG" and d["hungarian_method"] == "SCIPY"
+    assert Config.from_dict(d) == c
+    # a dict written by the reference (no extension keys) loads too
+    ref = {k: v for k, v in d.items() if k not in ("sinkhorn_fused", "gradient_precision")}
+    assert Config.from_dict(ref) == c
+
+
+def test_cpu_device_is_rejected():
+    from cugal_amd import cugal
+
+    with pytest.raises(NotImplementedError, match="ROCm device"):
+        cugal(nx.path_graph(4), nx.path_graph(4), Config())
+
+
+def test_synthetic_noise_free_target_is_permuted_source():
+    from cugal_amd.synthetic import erdos_renyi
+
+    inst = erdos_renyi(200, 0.05, 0.0, seed=3)
+    m = inst.source_mapping
+    src = {tuple(sorted((m[a], m[b]))) for a, b in inst.source.edges()}
+    tgt = {tuple(sorted(e)) for e in inst.target.edges()}
+    assert src == tgt
+
+
+def test_synthetic_noise_counts_and_determinism():
+    from cugal_amd.synthetic import barabasi_albert
+
+    a = barabasi_albert(500, 3, 0.05, seed=1)
+    b = barabasi_albert(500, 3, 0.05, seed=1)
+    assert sorted(a.target.edges()) == sorted(b.target.edges())
+    # refill keeps the edge count; removal never isolates a node
+    assert a.target.number_of_edges() == a.source.number_of_edges()
+    assert min(d for _, d in a.target.degree()) >= 1
+    changed = len({tuple(sorted(e)) for e in a.target.edges()}
+                  - {tuple(sorted((a.source_mapping[x], a.source_mapping[y]))) for x, y in a.source.edges()})
+    assert 0.02 * a.source.number_of_edges() < changed < 0.08 * a.source.number_of_edges()
+
+
+def test_oracle_not_imported_by_library():
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "cugal_amd")):
+        for f in files:
+            if f.endswith(".py"):
+                tree = ast.parse(open(os.path.join(dirpath, f)).read())
+                for node in ast.walk(tree):
+                    if isinstance(node, (ast.Import, ast.ImportFrom)):
+                        mod = getattr(node, "module", None) or ""
+                        names = [a.name for a in node.names]
+                        assert not mod.startswith("oracle") and not any(n.startswith("oracle") for n in names), f

@@ -1,0 +1,291 @@
+"""Pins of the FP64 oracle (oracle/fugal.py) against facts that do not come
+from the oracle itself: closed forms, brute force, networkx and scipy."""
+
+import itertools
+
+import networkx as nx
+import numpy as np
+import pytest
+import torch
+
+from oracle import fugal as oracle
+
+
+def _adj(g):
+    return nx.to_numpy_array(g, nodelist=sorted(g.nodes()), weight=None)
+
+
+# ---------------------------------------------------------------- features --
+
+@pytest.mark.parametrize("n", [3, 5, 9])
+def test_features_complete_graph(n):
+    f = oracle.extract_features_nx(nx.complete_graph(n))
+    np.testing.assert_array_equal(f, np.tile([n - 1, 1.0, n - 1, 1.0], (n, 1)))
+
+
+def test_features_cycle_and_star():
+    f = oracle.extract_features_nx(nx.cycle_graph(7))
+    np.testing.assert_array_equal(f, np.tile([2, 0.0, 2, 0.0], (7, 1)))
+    s = oracle.extract_features_nx(nx.star_graph(5))  # centre 0, leaves 1..5
+    np.testing.assert_array_equal(s[0], [5, 0.0, 1.0, 0.0])
+    np.testing.assert_array_equal(s[1:], np.tile([1, 0.0, 5.0, 0.0], (5, 1)))
+
+
+def test_features_isolated_node_is_zero():
+    g = nx.path_graph(4)
+    g.add_node(4)
+    f = oracle.extract_features_nx(g)
+    np.testing.assert_array_equal(f[4], [0, 0, 0, 0])
+
+
+def _brute_triangles(A):
+    n = A.shape[0]
+    t = np.zeros(n, dtype=np.int64)
+    for a, b, c in itertools.combinations(range(n), 3):
+        if A[a, b] and A[b, c] and A[a, c]:
+            t[[a, b, c]] += 1
+    return t
+
+
+@pytest.mark.parametrize("g", [nx.karate_club_graph(), nx.petersen_graph(),
+                               nx.erdos_renyi_graph(40, 0.2, seed=1), nx.watts_strogatz_graph(30, 6, 0.3, seed=2)])
+def test_triangles_and_clustering_brute_force(g):
+    A = _adj(g)
+    tri = _brute_triangles(A)
+    assert np.array_equal(oracle.triangle_counts(A), 2 * tri)
+    d = A.sum(1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        c = np.where(d >= 2, 2 * tri / (d * (d - 1)), 0.0)
+    f = oracle.extract_features_nx(g)
+    np.testing.assert_allclose(f[:, 1], c, rtol=1e-15, atol=0)
+    # neighbour means from adjacency products
+    with np.errstate(invalid="ignore", divide="ignore"):
+        nd = np.where(d > 0, (A @ d) / d, 0.0)
+        nc = np.where(d > 0, (A @ c) / d, 0.0)
+    np.testing.assert_allclose(f[:, 2], nd, rtol=1e-14)
+    np.testing.assert_allclose(f[:, 3], nc, rtol=1e-14, atol=1e-15)
+
+
+def test_karate_and_petersen_totals():
+    assert oracle.triangle_counts(_adj(nx.karate_club_graph())).sum() == 6 * 45
+    assert oracle.triangle_counts(_adj(nx.petersen_graph())).sum() == 0
+
+
+@pytest.mark.parametrize("g", [nx.karate_club_graph(), nx.erdos_renyi_graph(50, 0.15, seed=4)])
+def test_egonet_edges_is_degree_plus_triangles(g):
+    A = _adj(g)
+    assert np.array_equal(oracle.egonet_edges(g), A.sum(1).astype(np.int64) + _brute_triangles(A))
+
+
+def test_distance_matrix_direct_form():
+    g = torch.Generator().manual_seed(0)
+    a = torch.rand((60, 4), generator=g, dtype=torch.float64) * 7
+    b = torch.rand((60, 4), generator=g, dtype=torch.float64) * 7
+    direct = np.sqrt(((a.numpy()[:, None, :] - b.numpy()[None, :, :]) ** 2).sum(-1))
+    np.testing.assert_allclose(oracle.distance_matrix(a, b).numpy(), direct, atol=1e-9)
+
+
+# ---------------------------------------------------------------- gradient --
+
+def _pair(seed, n=30):
+    ga = nx.erdos_renyi_graph(n, 0.2, seed=seed)
+    gb = nx.erdos_renyi_graph(n, 0.2, seed=seed + 1)
+    return torch.tensor(_adj(ga)), torch.tensor(_adj(gb))
+
+
+def test_gradient_closed_form_uniform():
+    A, B = _pair(1)
+    n = A.shape[0]
+    P = torch.full((n, n), 1.0 / n, dtype=torch.float64)
+    G = oracle.sparse_gradient(A, B, P, torch.zeros_like(P), 0)
+    da, db = A.sum(1), B.sum(1)
+    np.testing.assert_allclose(G.numpy(), (-2 * torch.outer(da, db) / n).numpy(), atol=1e-12)
+
+
+def test_gradient_at_permutation_brute_force():
+    # (A Pi B)_ij = sum_k A_ik B_{pi(k) j}: count of neighbours k of i whose
+    # image pi(k) is a neighbour of j.
+    A, B = _pair(3, n=12)
+    n = A.shape[0]
+    perm = np.random.RandomState(0).permutation(n)
+    Pi = torch.zeros((n, n), dtype=torch.float64)
+    Pi[torch.arange(n), torch.tensor(perm)] = 1
+    G = oracle.sparse_gradient(A, B, Pi, torch.zeros_like(Pi), 0)
+    want = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            want[i, j] = -2 * sum(1 for k in range(n) if A[i, k] and B[perm[k], j])
+    np.testing.assert_array_equal(G.numpy(), want)
+
+
+def test_dense_minus_sparse_is_one_regulariser():
+    A, B = _pair(5)
+    n = A.shape[0]
+    P = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    D = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    lam = 3
+    diff = oracle.dense_gradient(A, B, P, D, lam) - oracle.sparse_gradient(A, B, P, D, lam)
+    np.testing.assert_allclose(diff.numpy(), (lam * (1 - 2 * P)).numpy(), atol=1e-12)
+
+
+# ---------------------------------------------------------------- sinkhorn --
+
+def test_sinkhorn_separable_cost_uniform_plan():
+    n = 40
+    g = torch.Generator().manual_seed(0)
+    C = torch.rand(n, 1, generator=g, dtype=torch.float64) * 3 + torch.rand(1, n, generator=g, dtype=torch.float64) * 3
+    for method in ("LOG", "STANDARD"):
+        st = oracle.SinkhornState(n, method)
+        if method == "LOG":
+            K, u, v, _ = st.solve_log(C, 0.5, 3, 0.0, 10)
+            plan = oracle.plan_log(K, u, v)
+        else:
+            K, u, v, _ = st.solve_standard(C, 0.5, 3, 0.0, 10)
+            plan = oracle.plan_standard(K, u, v)
+        np.testing.assert_allclose(plan.numpy(), np.full((n, n), 1.0 / n), rtol=1e-12)
+
+
+def test_sinkhorn_two_by_two_closed_form():
+    # C = [[0, c], [c, 0]]: K = [[1, e], [e, 1]], e = exp(-c/reg), whose
+    # doubly stochastic scaling is K / (1 + e).
+    c, reg = 1.3, 0.7
+    C = torch.tensor([[0.0, c], [c, 0.0]], dtype=torch.float64)
+    st = oracle.SinkhornState(2, "LOG")
+    K, u, v, _ = st.solve_log(C, reg, 50, 0.0, 10)
+    e = np.exp(-c / reg)
+    np.testing.assert_allclose(oracle.plan_log(K, u, v).numpy(), np.array([[1, e], [e, 1]]) / (1 + e), rtol=1e-12)
+
+
+def test_log_and_standard_agree_and_are_doubly_stochastic():
+    n = 64
+    C = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(3)) * 5
+    a = oracle.SinkhornState(n, "LOG")
+    K1, u1, v1, r1 = a.solve_log(C, 1.0, 500, 1e-9, 10)
+    b = oracle.SinkhornState(n, "STANDARD")
+    K2, u2, v2, r2 = b.solve_standard(C, 1.0, 500, 1e-9, 10)
+    p1, p2 = oracle.plan_log(K1, u1, v1), oracle.plan_standard(K2, u2, v2)
+    np.testing.assert_allclose(p1.numpy(), p2.numpy(), rtol=1e-7, atol=1e-12)
+    for p in (p1, p2):
+        np.testing.assert_allclose(p.sum(0).numpy(), np.ones(n), atol=1e-8)
+        np.testing.assert_allclose(p.sum(1).numpy(), np.ones(n), atol=1e-8)
+    assert r1.errors[-1] < 1e-9 and r2.errors[-1] < 1e-9
+
+
+def test_small_regularisation_approaches_lap():
+    import scipy.optimize
+
+    n = 12
+    C = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(7))
+    _, col = scipy.optimize.linear_sum_assignment(C.numpy())
+    st = oracle.SinkhornState(n, "LOG")
+    K, u, v, _ = st.solve_log(C, 0.005, 5000, 1e-10, 10)
+    plan = oracle.plan_log(K, u, v)
+    assert np.array_equal(plan.argmax(1).numpy(), col)
+
+
+def test_log_marginal_quirk_corrupts_kernel():
+    # The reference's in-place marginal check (cugal/sinkhorn.py:67-74) leaves
+    # K changed; the default path does not.
+    n = 10
+    C = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    st = oracle.SinkhornState(n, "LOG")
+    K = -C / 1.0
+    before = K.clone()
+    st.marginal_error_log(K, quirk=False)
+    assert torch.equal(K, before)
+    st.marginal_error_log(K, quirk=True)
+    assert not torch.equal(K, before)
+
+
+def test_lehmann_momentum_rule():
+    st = oracle.SinkhornState(4, "LOG")
+    assert st.lehmann_momentum(None, 10, [1.0, 0.5]) == 1.0
+    assert st.lehmann_momentum(40, 10, [1.0, 0.5]) == 1.0  # 2*10 < 40
+    w = st.lehmann_momentum(20, 10, [1.0, 0.5])            # ratio 0.5 / 1.0 = 0.5
+    assert w == pytest.approx(2.0 / (1.0 + np.sqrt(1 - 0.5 ** 0.1)))
+    assert 1.0 < w < 2.0
+
+
+# ------------------------------------------------------------- update, LAP --
+
+def test_update_alpha_limits():
+    n = 8
+    P = torch.rand((n, n), dtype=torch.float64)
+    plan = torch.rand((n, n), dtype=torch.float64)
+    Q = P.clone()
+    oracle.update_quasi_permutation(Q, plan, 1.0)
+    assert torch.equal(Q, plan)
+    Q = P.clone()
+    oracle.update_quasi_permutation(Q, plan, 0.0)
+    assert torch.equal(Q, P)
+
+
+def test_update_keeps_doubly_stochastic():
+    n = 16
+    st = oracle.SinkhornState(n, "LOG")
+    C = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    K, u, v, _ = st.solve_log(C, 0.5, 1000, 1e-12, 10)
+    P = torch.full((n, n), 1.0 / n, dtype=torch.float64)
+    oracle.update_quasi_permutation(P, oracle.plan_log(K, u, v), 2.0 / 3.0)
+    np.testing.assert_allclose(P.sum(0).numpy(), np.ones(n), atol=1e-9)
+    np.testing.assert_allclose(P.sum(1).numpy(), np.ones(n), atol=1e-9)
+
+
+def _brute_lap(M):
+    n = M.shape[0]
+    best, arg = -np.inf, None
+    for perm in itertools.permutations(range(n)):
+        s = M[np.arange(n), perm].sum()
+        if s > best:
+            best, arg = s, perm
+    return np.array(arg)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_scipy_lap_brute_force(seed):
+    M = torch.rand((7, 7), dtype=torch.float64, generator=torch.Generator().manual_seed(seed))
+    assert np.array_equal(oracle.scipy_lap(M), _brute_lap(M.numpy()))
+
+
+@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY"])
+def test_greedy_optimal_on_dominant_matrix(method):
+    # With one entry per row larger than the rest of its row and column,
+    # greedy is optimal, so it equals the brute-force LAP.
+    n = 7
+    g = torch.Generator().manual_seed(1)
+    perm = torch.randperm(n, generator=g)
+    M = torch.rand((n, n), generator=g, dtype=torch.float64) * 0.1
+    M[torch.arange(n), perm] = 1.0 + torch.rand(n, generator=g, dtype=torch.float64)
+    want = _brute_lap(M.numpy())
+    assert oracle.greedy_lap(M.clone(), method) == list(want)
+
+
+def test_double_greedy_order_and_masking():
+    # Row 1 holds the largest maximum, so it is served first and takes
+    # column 0; row 0 then sees column 0 masked to 0 and takes column 1.
+    M = torch.tensor([[0.5, 0.4, 0.1], [0.9, 0.05, 0.05], [0.2, 0.3, 0.45]], dtype=torch.float64)
+    Q = M.clone()
+    assert oracle.greedy_lap(Q, "DOUBLE_GREEDY") == [1, 0, 2]
+    assert Q[0, 0] == 0 and Q[1, 0] == 0.9          # masking is in place, taken cols only
+    assert oracle.greedy_lap(M.clone(), "GREEDY") == [0, 1, 2]
+
+
+# ------------------------------------------------------------- pipeline --
+
+def test_pipeline_recovers_isomorphic_pair():
+    from cugal_amd.synthetic import alignment_accuracy, answer_columns, watts_strogatz
+
+    inst = watts_strogatz(80, 6, 0.3, 0.0, seed=5)
+    run = oracle.align(inst.source, inst.target, oracle.OracleConfig(
+        sinkhorn_regularization=0.3, sinkhorn_method="LOG", mu=2.0, iter_count=5, frank_wolfe_iter_count=5,
+        use_sparse_adjacency=True))
+    assert alignment_accuracy(inst.source_mapping, answer_columns(run.mapping)) == 1.0
+
+
+def test_pipeline_default_config_small():
+    # Reference default Config (STANDARD, FP64, dense quirk, DOUBLE_GREEDY).
+    from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi
+
+    inst = erdos_renyi(60, 0.1, 0.0, seed=2)
+    run = oracle.align(inst.source, inst.target, oracle.OracleConfig(iter_count=4, frank_wolfe_iter_count=4))
+    assert alignment_accuracy(inst.source_mapping, answer_columns(run.mapping)) >= 0.9
