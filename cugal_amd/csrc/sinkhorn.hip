@@ -331,44 +331,53 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
     const void* __restrict__ part_raw, int nblk, int n, const float* __restrict__ v2_in, float* __restrict__ v2_out,
     int raw_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
-    __shared__ float2 red[kCombWaves][kCombCols];
+    __shared__ float rm[kCombWaves][kCombCols];
+    __shared__ double rs[kCombWaves][kCombCols];
     __shared__ double dred[kCombWaves];
     __shared__ int is_last;
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
     const int j = blockIdx.x * kCombCols + lane;
-    Lse a = {0.0f, 0.0f};
+    // FP64 accumulation: the column sums feed the convergence test.
+    float m = 0.0f;
+    double sum = 0.0;
     if (PAIRS) {
-        a.m = kNegBig;
         const float2* part = static_cast<const float2*>(part_raw);
-        if (j < n)
+        m = kNegBig;
+        if (j < n) {
+            for (int b = wave; b < nblk; b += kCombWaves) m = fmaxf(m, part[static_cast<int64_t>(b) * n + j].x);
             for (int b = wave; b < nblk; b += kCombWaves) {
                 const float2 p = part[static_cast<int64_t>(b) * n + j];
-                a = lse_merge(a, {p.x, p.y});
+                sum += static_cast<double>(p.y) * exp2(static_cast<double>(p.x) - static_cast<double>(m));
             }
+        }
     } else {
         const float* part = static_cast<const float*>(part_raw);
         if (j < n)
-            for (int b = wave; b < nblk; b += kCombWaves) a.s += part[static_cast<int64_t>(b) * n + j];
+            for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
     }
-    red[wave][lane] = make_float2(a.m, a.s);
+    rm[wave][lane] = m;
+    rs[wave][lane] = sum;
     __syncthreads();
     double err = 0.0;
     if (wave == 0 && j < n) {
-        float L;
+        double L;
         if (PAIRS) {
-            Lse t = {kNegBig, 0.0f};
-            for (int w = 0; w < kCombWaves; ++w) t = lse_merge(t, {red[w][lane].x, red[w][lane].y});
-            L = t.m + log2_fast(t.s);
+            float M = kNegBig;
+            for (int w = 0; w < kCombWaves; ++w) M = fmaxf(M, rm[w][lane]);
+            double S = 0.0;
+            for (int w = 0; w < kCombWaves; ++w)
+                S += rs[w][lane] * exp2(static_cast<double>(rm[w][lane]) - static_cast<double>(M));
+            L = static_cast<double>(M) + log2(S);
         } else {
-            float S = 0.0f;
-            for (int w = 0; w < kCombWaves; ++w) S += red[w][lane].y;
-            L = log2_fast(S);
+            double S = 0.0;
+            for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+            L = log2(S);
         }
-        const float vj = v2_in[j];
-        v2_out[j] = raw_out ? (L - vj) : (vj - L);
-        if (want_err) err = fabs(exp2(static_cast<double>(L)) - 1.0);
+        const double vj = static_cast<double>(v2_in[j]);
+        v2_out[j] = static_cast<float>(raw_out ? (L - vj) : (vj - L));
+        if (want_err) err = fabs(exp2(L) - 1.0);
     }
     if (!want_err) return;
     // Block sum of the error (wave 0 holds all terms).

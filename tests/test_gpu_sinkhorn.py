@@ -43,6 +43,22 @@ U_ATOL = 5e-5
 PLAN_RTOL = 1e-4
 
 
+def error_floor(n, C, reg):
+    """FP32 floor of the column-marginal L1 error: each column sum carries
+    the rounding of exponents of magnitude up to |C|/reg * log2(e) (unit
+    roundoff 2^-24 each), so sum_j |c_j - 1| cannot be resolved below about
+    n * 2^-24 * (4 + max|K2|).  The FP64 oracle reaches ~1e-13; the FP32
+    kernels stop improving at this floor (the reference's FP32 kernels do
+    too)."""
+    kmax = float(C.abs().max()) / reg * 1.4426950408889634
+    return n * 2.0 ** -24 * (4.0 + kmax)
+
+
+def assert_errors_close(got, want, n, C, reg, rtol=2e-3):
+    assert len(got) == len(want)
+    np.testing.assert_allclose(got, want, rtol=rtol, atol=error_floor(n, C, reg))
+
+
 @pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
 @pytest.mark.parametrize("fused", [True, False])
 def test_fixed_iterations_match_oracle(device, n, fused):
@@ -58,7 +74,7 @@ def test_fixed_iterations_match_oracle(device, n, fused):
     np.testing.assert_allclose(plan.numpy(), ref_plan.numpy(), rtol=PLAN_RTOL * 10, atol=1e-9)
     # errors recorded at iterations 10, 20, 30 (not 0)
     assert len(errors) == len(res.errors) == 3
-    np.testing.assert_allclose(errors, res.errors, rtol=2e-3, atol=1e-6)
+    assert_errors_close(errors, res.errors, n, C, reg)
 
 
 def test_fused_equals_two_sweep(device):
@@ -76,6 +92,7 @@ def test_threshold_stop_and_warm_start(device):
     C2 = C1 + 0.05 * _cost(n, seed=12, scale=1.0)
     u, v, e1, c1 = _run_gpu(C1, 0.5, 500, 1e-3, 10, True, device)
     K, ru, rv, r1 = _run_oracle(C1, 0.5, 500, 1e-3, 10)
+    assert error_floor(n, C1, 0.5) < 1e-3  # the threshold is resolvable in FP32 here
     assert c1 == r1.iteration_count
     assert e1[-1] < 1e-3 and len(e1) == len(r1.errors)
     # second solve warm-starts from the first one's duals
@@ -87,13 +104,20 @@ def test_threshold_stop_and_warm_start(device):
     np.testing.assert_allclose(u2.numpy(), ru2.numpy(), atol=1e-4)
 
 
-def test_momentum_matches_oracle(device):
-    n = 300
-    C = _cost(n, seed=5, scale=4.0)
-    u, v, e, c = _run_gpu(C, 0.4, 80, 0.0, 10, True, device, momentum_start=20)
-    K, ru, rv, res = _run_oracle(C, 0.4, 80, 0.0, 10, momentum_start=20)
-    np.testing.assert_allclose(e, res.errors, rtol=5e-3, atol=1e-6)
-    np.testing.assert_allclose(u.numpy(), ru.numpy(), atol=2e-4)
+@pytest.mark.parametrize("fused", [True, False])
+def test_momentum_matches_oracle(device, fused):
+    # Errors at iterations 10 and 20 (0.28 and 4.7e-3 in FP64) sit above the
+    # FP32 floor, so the Lehmann factor computed from their ratio is the
+    # oracle's; later errors reach the floor.
+    n, reg = 300, 0.05
+    C = _cost(n, seed=5, scale=10.0)
+    u, v, e, c = _run_gpu(C, reg, 80, 0.0, 10, fused, device, momentum_start=20)
+    K, ru, rv, res = _run_oracle(C, reg, 80, 0.0, 10, momentum_start=20)
+    assert e[0] > 0.1 and e[1] > 1e-3
+    assert_errors_close(e, res.errors, n, C, reg, rtol=2e-2)
+    plan = oracle.plan_log(K, u, v)
+    ref_plan = oracle.plan_log(K, ru, rv)
+    np.testing.assert_allclose(plan.numpy(), ref_plan.numpy(), rtol=2e-3, atol=1e-7)
 
 
 def test_separable_cost_gives_uniform_plan(device):
