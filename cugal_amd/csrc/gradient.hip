@@ -6,14 +6,16 @@
 // read from an n x n matrix (cuda_kernels/distance.cu:11-33 computes the same
 // quantity; cugal/feature_extraction.py:98-108).
 //
-// Two passes, each reading and writing n x n once:
-//   1. T = P B, row-local: rows of P are staged in LDS and every output
-//      column j gathers P[i][l] for l in N_B(j) (B symmetric, so the CSR row
-//      j lists the non-zeros of column j).
-//   2. G = epilogue(-2 A T): output row i sums the rows T[k] for k in N_A(i),
-//      float4 per lane; tiles are remapped so that consecutive rows of one
-//      column chunk land on the same XCD and share its L2 for common
-//      neighbours.
+// Both sparse products are row gathers (out row r = sum of the input rows
+// listed by CSR row r), which read whole 256-byte row segments; the column
+// side is turned into a row side by transposing through LDS:
+//   1. Y = (A P)^T : 64 x 64 tiles of A P gathered, stored transposed;
+//   2. G = epilogue(-2 (B Y)^T): tiles of B Y gathered, transposed in LDS
+//      and written as rows of G with the distance and regulariser terms
+//      (B symmetric, so (B Y)^T = (A P) B).
+// Tiles are dealt so that the blocks one XCD runs together cover
+// consecutive row tiles of one column strip and share that XCD's L2 for
+// common neighbours.
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -23,51 +25,7 @@ namespace {
 constexpr int kSpThreads = 256;
 constexpr int kNumXcd = 8;
 
-// ---- pass 1: out[i][j] = sum_{l in N(j)} X[i][l] -------------------------
-template <int R>
-__global__ __launch_bounds__(kSpThreads) void spmm_rowlocal_lds(const int32_t* __restrict__ bcol,
-                                                               const int32_t* __restrict__ brp, int n,
-                                                               const float* __restrict__ X, float* __restrict__ out) {
-    extern __shared__ float sx[];
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * R;
-    const int rows = static_cast<int>(min(static_cast<int64_t>(R), n - r0));
-    for (int r = 0; r < rows; ++r) {
-        const float* src = X + (r0 + r) * n;
-        for (int j = threadIdx.x; j < n; j += kSpThreads) sx[r * n + j] = src[j];
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += kSpThreads) {
-        const int b = brp[j];
-        const int e = brp[j + 1];
-        float acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = 0.0f;
-        for (int k = b; k < e; ++k) {
-            const int l = bcol[k];
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] += sx[r * n + l];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (r < rows) out[(r0 + r) * n + j] = acc[r];
-    }
-}
-
-// Same without LDS staging (rows wider than the LDS).
-__global__ __launch_bounds__(kSpThreads) void spmm_rowlocal_global(const int32_t* __restrict__ bcol,
-                                                                  const int32_t* __restrict__ brp, int n,
-                                                                  const float* __restrict__ X,
-                                                                  float* __restrict__ out) {
-    const int64_t i = blockIdx.x;
-    const float* src = X + i * n;
-    for (int j = threadIdx.x; j < n; j += kSpThreads) {
-        float acc = 0.0f;
-        for (int k = brp[j]; k < brp[j + 1]; ++k) acc += src[bcol[k]];
-        out[i * n + j] = acc;
-    }
-}
-
-// ---- pass 2: out[i][c] = scale * sum_{k in N(i)} X[k][c] (+ epilogue) -----
+// ---- row gather out[i][c] = scale * sum_{k in N(i)} X[k][c] (+ epilogue) --
 struct Epilogue {
     const float* P;   // nullptr: no regulariser term
     const float* fs;  // source features n x 4 (nullptr: no distance term)
@@ -136,6 +94,81 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
     }
 }
 
+// ---- tiled gather with a transposed store ---------------------------------
+// For the 64 x 64 tile (rows r0.., columns c0..) of Z = M X (M given by CSR):
+//   Z[r][c] = sum_{k in N(r)} X[k][c],
+// write out[c][r] = Z[r][c] (EPI == false), or, with EPI, treat (c, r) as
+// (i, j) of the gradient: out[c][r] = scale * Z[r][c] + D(c, r) + rl (1 - 2 P[c][r]).
+constexpr int kTile = 64;
+
+template <bool EPI, bool VEC>
+__global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ col, int n,
+                                                           const float* __restrict__ X, float scale, Epilogue ep,
+                                                           float* __restrict__ out, int tiles_r, int64_t tiles) {
+    __shared__ float tile[kTile][kTile + 1];
+    const int64_t t = xcd_tile(blockIdx.x, tiles);
+    if (t >= tiles) return;
+    const int r0 = static_cast<int>(t % tiles_r) * kTile;
+    const int c0 = static_cast<int>(t / tiles_r) * kTile;
+    const int g = threadIdx.x >> 4;   // 16 groups of 16 threads
+    const int q = threadIdx.x & 15;   // float4 slot within a 64-float segment
+    const int cq = c0 + 4 * q;
+    for (int rr = g; rr < kTile; rr += 16) {
+        const int r = r0 + rr;
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < n) {
+            const int b = rp[r];
+            const int e = rp[r + 1];
+            if (VEC) {
+                if (cq < n) {
+                    for (int k = b; k < e; ++k) {
+                        const float4 x = *reinterpret_cast<const float4*>(X + static_cast<int64_t>(col[k]) * n + cq);
+                        acc.x += x.x;
+                        acc.y += x.y;
+                        acc.z += x.z;
+                        acc.w += x.w;
+                    }
+                }
+            } else {
+                for (int k = b; k < e; ++k) {
+                    const float* xr = X + static_cast<int64_t>(col[k]) * n;
+                    if (cq + 0 < n) acc.x += xr[cq + 0];
+                    if (cq + 1 < n) acc.y += xr[cq + 1];
+                    if (cq + 2 < n) acc.z += xr[cq + 2];
+                    if (cq + 3 < n) acc.w += xr[cq + 3];
+                }
+            }
+        }
+        tile[rr][4 * q + 0] = acc.x;
+        tile[rr][4 * q + 1] = acc.y;
+        tile[rr][4 * q + 2] = acc.z;
+        tile[rr][4 * q + 3] = acc.w;
+    }
+    __syncthreads();
+    const int rq = r0 + 4 * q;  // four consecutive output columns
+    for (int cc = g; cc < kTile; cc += 16) {
+        const int c = c0 + cc;
+        if (c >= n) break;
+        float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
+        const int64_t base = static_cast<int64_t>(c) * n;
+        if (EPI) {
+            const float4 fsi = ep.fs ? *reinterpret_cast<const float4*>(ep.fs + 4 * static_cast<int64_t>(c))
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (rq + e < n) v[e] = epi_one(v[e], scale, ep, base + rq + e, fsi, rq + e);
+        }
+        if (VEC && rq + 3 < n) {
+            *reinterpret_cast<float4*>(out + base + rq) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (rq + e < n) out[base + rq + e] = v[e];
+        }
+    }
+}
+
 // ---- elementwise reference-compatible helpers ----------------------------
 __global__ void add_distance_kernel(const float* __restrict__ s, const float* __restrict__ t, int n,
                                     float* __restrict__ out) {
@@ -154,7 +187,7 @@ __global__ void regularize_kernel(float* __restrict__ g, const float* __restrict
     g[idx] += lam - lam * 2.0f * P[idx];
 }
 
-int max_lds_bytes() {
+[[maybe_unused]] int max_lds_bytes() {
     static int cached = -1;
     if (cached < 0) {
         int dev = 0, v = 0;
@@ -166,31 +199,22 @@ int max_lds_bytes() {
     return cached;
 }
 
-template <int R>
-int launch_rowlocal_r(const int32_t* bcol, const int32_t* brp, int n, const float* X, float* out, hipStream_t s) {
-    const size_t lds = sizeof(float) * static_cast<size_t>(R) * n;
-    static bool attr_set = false;
-    if (!attr_set && lds > 65536) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&spmm_rowlocal_lds<R>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(spmm_rowlocal_lds<R>, dim3(static_cast<unsigned>(div_up(n, R))), dim3(kSpThreads), lds, s,
-                       bcol, brp, n, X, out);
-    return check_launch("spmm_rowlocal_lds");
-}
-
-int launch_rowlocal(const int32_t* bcol, const int32_t* brp, int n, const float* X, float* out, hipStream_t s) {
-    const int64_t budget = max_lds_bytes();
-    const int64_t row_bytes = 4 * static_cast<int64_t>(n);
-    const int64_t fit = budget / row_bytes;
-    if (fit >= 8) return launch_rowlocal_r<8>(bcol, brp, n, X, out, s);
-    if (fit >= 4) return launch_rowlocal_r<4>(bcol, brp, n, X, out, s);
-    if (fit >= 2) return launch_rowlocal_r<2>(bcol, brp, n, X, out, s);
-    if (fit >= 1) return launch_rowlocal_r<1>(bcol, brp, n, X, out, s);
-    hipLaunchKernelGGL(spmm_rowlocal_global, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, bcol, brp, n,
-                       X, out);
-    return check_launch("spmm_rowlocal_global");
+template <bool EPI>
+int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, float scale, const Epilogue& ep,
+                       float* out, hipStream_t s) {
+    const bool vec = (n % 4 == 0) && aligned16(X) && aligned16(out) && (!ep.P || aligned16(ep.P)) &&
+                     (!ep.fs || (aligned16(ep.fs) && aligned16(ep.ft)));
+    const int tiles_r = static_cast<int>(div_up(n, kTile));
+    const int64_t tiles = static_cast<int64_t>(tiles_r) * tiles_r;
+    const int64_t blocks = div_up(tiles, kNumXcd) * kNumXcd;
+    if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
+    if (vec)
+        hipLaunchKernelGGL((gather_tile_t<EPI, true>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
+                           col, n, X, scale, ep, out, tiles_r, tiles);
+    else
+        hipLaunchKernelGGL((gather_tile_t<EPI, false>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
+                           rp, col, n, X, scale, ep, out, tiles_r, tiles);
+    return check_launch("gather_tile_t");
 }
 
 int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float* X, float scale, const Epilogue& ep,
@@ -261,11 +285,13 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     if (!workspace || workspace_bytes < cg_gradient_workspace(n))
         return fail(CG_EWORKSPACE, "cg_gradient: workspace too small");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    float* T = static_cast<float*>(workspace);
+    float* Y = static_cast<float*>(workspace);
     const int ni = static_cast<int>(n);
-    if (int rc = launch_rowlocal(b_col, b_rowptr, ni, P, T, s)) return rc;
-    Epilogue ep{P, source_features, target_features, regularizer_factor * static_cast<float>(lambda_)};
-    return launch_rowgather(a_col, a_rowptr, ni, T, -2.0f, ep, gradient, s);
+    const Epilogue none{nullptr, nullptr, nullptr, 0.0f};
+    if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, P, 1.0f, none, Y, s)) return rc;
+    const float rl = regularizer_factor * static_cast<float>(lambda_);
+    Epilogue ep{rl != 0.0f ? P : nullptr, source_features, target_features, rl};
+    return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, -2.0f, ep, gradient, s);
 }
 
 }  // extern "C"

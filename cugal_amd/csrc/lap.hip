@@ -1,55 +1,119 @@
 // Greedy rounding of the quasi-permutation to a permutation,
 // cugal/greedy_lap.py:7-63 (GREEDY :10-19, DOUBLE_GREEDY :50-63).
 //
-// DOUBLE_GREEDY: rows are visited by descending row maximum (a stable LSD
-// radix sort on the order-preserving key, so equal maxima keep row order;
-// the reference's torch.argsort is not stable).  For each visited row r the
-// reference multiplies P[r] by the 0/1 "not taken" vector in place and takes
-// argmax (first index of the maximum).  Both steps are reproduced exactly:
-// the taken entries of row r become P[r][j] * 0 in memory, and ties go to the
-// lowest column.  The walk is sequential by definition; it runs in a single
-// persistent workgroup with the taken set as a bitmap in LDS.
+// Reference semantics: rows are visited in order (GREEDY: 0..n-1;
+// DOUBLE_GREEDY: by descending row maximum); each visited row r is
+// multiplied in place by the 0/1 "not taken" vector and the first index of
+// its maximum is taken.
+//
+// MI355X form (exact, not a heuristic):
+//   1. row_topk: every row's 16 largest entries ordered by (value desc,
+//      index asc), one wave per row, fully parallel;
+//   2. DOUBLE_GREEDY only: a stable descending sort of the row maxima (the
+//      reference's argsort is not stable: with tied maxima it may visit rows
+//      in another order);
+//   3. walk: one wave visits the rows in order.  The first candidate not yet
+//      taken is the argmax of the masked row whenever its value is > 0: every
+//      non-candidate is after it in (value desc, index asc) order and masked
+//      columns read 0.  Otherwise (all 16 taken, or a non-positive best) the
+//      wave scans the full masked row.  The next row's candidates are loaded
+//      while the current one is decided.
+//   4. mask: P[r][j] *= 0 for every column j taken before row r was visited,
+//      which is what the in-place masking leaves in the matrix.
 #include "common.hpp"
 #include "cugal_hip.h"
 
 namespace cg {
 namespace {
 
-constexpr int kLapThreads = 1024;
+constexpr int kTopK = 16;
 constexpr int kSortThreads = 1024;
 constexpr int64_t kLapMaxN = int64_t(1) << 20;  // 128 KiB taken bitmap
 
-__global__ __launch_bounds__(256) void row_max_kernel(const float* __restrict__ P, int n, float* __restrict__ rmax) {
+// (value, index) order: larger value first, then lower index.
+__device__ __forceinline__ bool before(float v, int i, float v2, int i2) { return v > v2 || (v == v2 && i < i2); }
+
+// ---- 1. per-row top-K ----------------------------------------------------
+__global__ __launch_bounds__(256) void row_topk(const float* __restrict__ P, int n, float* __restrict__ cval,
+                                               int32_t* __restrict__ cidx, float* __restrict__ rmax) {
     const int lane = threadIdx.x & 63;
     const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / 64;
     if (r >= n) return;
     const float* row = P + r * n;
-    float m = -INFINITY;
+    float lv[kTopK];
+    int li[kTopK];
+#pragma unroll
+    for (int k = 0; k < kTopK; ++k) {
+        lv[k] = -INFINITY;
+        li[k] = 0x7fffffff;
+    }
     bool nan = false;
+    // A lane sees its columns in increasing index order, so a new element
+    // enters only if strictly larger than the list's last (ties keep the
+    // lower index already held).
     for (int j = lane; j < n; j += 64) {
         const float x = row[j];
         nan |= (x != x);
-        m = fmaxf(m, x);
+        if (x > lv[kTopK - 1]) {
+            float cv = x;
+            int ci = j;
+#pragma unroll
+            for (int k = 0; k < kTopK; ++k) {
+                if (before(cv, ci, lv[k], li[k])) {
+                    const float tv = lv[k];
+                    const int ti = li[k];
+                    lv[k] = cv;
+                    li[k] = ci;
+                    cv = tv;
+                    ci = ti;
+                }
+            }
+        }
     }
-    m = wave_max(m);
-    // torch.max propagates NaN.
+    // Merge the 64 lane lists: kTopK rounds of a wave-wide best head.
+    int head = 0;
+    for (int k = 0; k < kTopK; ++k) {
+        float hv = -INFINITY;
+        int hi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < kTopK; ++q)
+            if (q == head) {
+                hv = lv[q];
+                hi = li[q];
+            }
+        float bv = hv;
+        int bi = hi;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ov = __shfl_xor(bv, off, kWave);
+            const int oi = __shfl_xor(bi, off, kWave);
+            if (before(ov, oi, bv, bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (hi == bi && bi != 0x7fffffff) ++head;  // the owner pops its head
+        if (lane == 0) {
+            cval[r * kTopK + k] = bv;
+            cidx[r * kTopK + k] = bi;
+        }
+    }
     const bool any_nan = __any(nan);
-    if (lane == 0) rmax[r] = any_nan ? NAN : m;
+    if (lane == 0 && rmax) rmax[r] = any_nan ? NAN : cval[r * kTopK];  // torch.max propagates NaN
 }
 
-// Order-preserving key of a float, inverted so that an ascending sort gives
-// descending values (NaN sorts first, as the largest).
+// ---- 2. stable sort of the row maxima (descending) -----------------------
 __device__ __forceinline__ uint32_t desc_key(float f) {
-    if (f != f) return 0u;
+    if (f != f) return 0u;  // NaN first, as the largest
     if (f == 0.0f) f = 0.0f;  // -0 and +0 tie, as in a float comparison
     uint32_t b = __float_as_uint(f);
     b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
     return ~b;
 }
 
-// Stable LSD radix sort, one bit per pass, in a single block.  Each thread
-// owns a contiguous slice, so a stable split of each slice plus a block scan
-// keeps the global order stable.
+// LSD radix sort, one bit per pass, in one block.  Each thread owns a
+// contiguous slice, so a stable split of each slice plus a block scan keeps
+// the global order stable.
 __global__ __launch_bounds__(kSortThreads) void stable_sort_desc(const float* __restrict__ vals, int n,
                                                                 uint32_t* __restrict__ ka, int32_t* __restrict__ ia,
                                                                 uint32_t* __restrict__ kb, int32_t* __restrict__ ib,
@@ -107,59 +171,97 @@ __global__ void identity_order(int32_t* __restrict__ order, int n) {
     if (i < n) order[i] = i;
 }
 
-// Argmax pair: larger value wins, equal values -> lower index.
-__device__ __forceinline__ void arg_better(float& v, int& i, float v2, int i2) {
-    if (v2 > v || (v2 == v && i2 < i)) {
-        v = v2;
-        i = i2;
-    }
-}
+// ---- 3. the sequential walk (one wave) -----------------------------------
+__device__ __forceinline__ bool is_taken(const uint32_t* taken, int j) { return (taken[j >> 5] >> (j & 31)) & 1u; }
 
-__global__ __launch_bounds__(kLapThreads) void greedy_walk(float* __restrict__ P, int n,
-                                                          const int32_t* __restrict__ order,
-                                                          int32_t* __restrict__ assignment) {
+__global__ __launch_bounds__(64) void greedy_walk(const float* __restrict__ P, int n, const int32_t* __restrict__ order,
+                                                 const float* __restrict__ cval, const int32_t* __restrict__ cidx,
+                                                 int32_t* __restrict__ assignment, int32_t* __restrict__ step_of_col,
+                                                 int32_t* __restrict__ step_of_row, int32_t* __restrict__ fallbacks) {
     extern __shared__ uint32_t taken[];  // n bits
-    __shared__ float wv[kLapThreads / kWave];
-    __shared__ int wi[kLapThreads / kWave];
+    const int lane = threadIdx.x;
     const int words = static_cast<int>(div_up(n, 32));
-    for (int w = threadIdx.x; w < words; w += kLapThreads) taken[w] = 0u;
+    for (int w = lane; w < words; w += 64) taken[w] = 0u;
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x / kWave;
+    int nfall = 0;
+    int r = order[0];
+    float v = lane < kTopK ? cval[static_cast<int64_t>(r) * kTopK + lane] : -INFINITY;
+    int c = lane < kTopK ? cidx[static_cast<int64_t>(r) * kTopK + lane] : 0x7fffffff;
     for (int k = 0; k < n; ++k) {
-        const int r = order[k];
-        float* row = P + static_cast<int64_t>(r) * n;
-        float bv = -INFINITY;
-        int bi = 0x7fffffff;
-        for (int j = threadIdx.x; j < n; j += kLapThreads) {
-            float x = row[j];
-            if ((taken[j >> 5] >> (j & 31)) & 1u) {
-                x = x * 0.0f;
-                row[j] = x;
+        // Prefetch the next row's candidates.
+        int rn = 0;
+        float vn = -INFINITY;
+        int cn = 0x7fffffff;
+        if (k + 1 < n) {
+            rn = order[k + 1];
+            if (lane < kTopK) {
+                vn = cval[static_cast<int64_t>(rn) * kTopK + lane];
+                cn = cidx[static_cast<int64_t>(rn) * kTopK + lane];
             }
-            // NaN never wins here (the reference's argmax would pick it).
-            arg_better(bv, bi, x, j);
         }
+        const bool free_c = lane < kTopK && c != 0x7fffffff && !is_taken(taken, c);
+        const unsigned long long m = __ballot(free_c);
+        int choice = -1;
+        if (m) {
+            const int first = __ffsll(m) - 1;
+            const float bv = __shfl(v, first, kWave);
+            const int bc = __shfl(c, first, kWave);
+            if (bv > 0.0f) choice = bc;
+        }
+        if (choice < 0) {
+            // Full scan of the masked row: taken columns read x * 0.
+            ++nfall;
+            const float* row = P + static_cast<int64_t>(r) * n;
+            float bv = -INFINITY;
+            int bi = 0x7fffffff;
+            for (int j = lane; j < n; j += 64) {
+                float x = row[j];
+                if (is_taken(taken, j)) x = x * 0.0f;
+                if (before(x, j, bv, bi)) {
+                    bv = x;
+                    bi = j;
+                }
+            }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const float ov = __shfl_xor(bv, off, kWave);
-            const int oi = __shfl_xor(bi, off, kWave);
-            arg_better(bv, bi, ov, oi);
+            for (int off = 32; off > 0; off >>= 1) {
+                const float ov = __shfl_xor(bv, off, kWave);
+                const int oi = __shfl_xor(bi, off, kWave);
+                if (before(ov, oi, bv, bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            choice = bi == 0x7fffffff ? 0 : bi;  // an all-NaN row: index 0
         }
         if (lane == 0) {
-            wv[wave] = bv;
-            wi[wave] = bi;
+            assignment[r] = choice;
+            step_of_row[r] = k;
+            if (step_of_col[choice] < 0) step_of_col[choice] = k;
+            taken[choice >> 5] |= 1u << (choice & 31);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            float v = wv[0];
-            int i = wi[0];
-            for (int w = 1; w < kLapThreads / kWave; ++w) arg_better(v, i, wv[w], wi[w]);
-            if (i == 0x7fffffff) i = 0;  // all entries NaN / -inf: argmax of torch returns index 0
-            assignment[r] = i;
-            taken[i >> 5] |= 1u << (i & 31);
-        }
-        __syncthreads();
+        r = rn;
+        v = vn;
+        c = cn;
+    }
+    if (lane == 0 && fallbacks) *fallbacks = nfall;
+}
+
+__global__ void fill_int(int32_t* __restrict__ p, int n, int value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// ---- 4. the in-place masking the reference leaves in P -------------------
+__global__ __launch_bounds__(256) void mask_taken(float* __restrict__ P, int n,
+                                                 const int32_t* __restrict__ step_of_col,
+                                                 const int32_t* __restrict__ step_of_row) {
+    const int64_t r = blockIdx.x;
+    const int s = step_of_row[r];
+    float* row = P + r * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int t = step_of_col[j];
+        if (t >= 0 && t < s) row[j] = row[j] * 0.0f;
     }
 }
 
@@ -173,12 +275,17 @@ extern "C" {
 size_t cg_lap_workspace(int64_t n) {
     if (n <= 0) return 0;
     Carver cv{nullptr, 0, ~size_t(0)};
+    cv.take<float>(n * kTopK);
+    cv.take<int32_t>(n * kTopK);
     cv.take<float>(n);
     cv.take<uint32_t>(n);
     cv.take<int32_t>(n);
     cv.take<uint32_t>(n);
     cv.take<int32_t>(n);
     cv.take<int32_t>(n);
+    cv.take<int32_t>(n);
+    cv.take<int32_t>(n);
+    cv.take<int32_t>(1);
     return cv.used + 256;
 }
 
@@ -191,22 +298,30 @@ int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* wo
     if (!workspace || workspace_bytes < cg_lap_workspace(n)) return fail(CG_EWORKSPACE, "cg_lap_greedy: workspace too small");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    float* cval = cv.take<float>(n * kTopK);
+    int32_t* cidx = cv.take<int32_t>(n * kTopK);
     float* rmax = cv.take<float>(n);
     uint32_t* ka = cv.take<uint32_t>(n);
     int32_t* ia = cv.take<int32_t>(n);
     uint32_t* kb = cv.take<uint32_t>(n);
     int32_t* ib = cv.take<int32_t>(n);
     int32_t* order = cv.take<int32_t>(n);
+    int32_t* step_col = cv.take<int32_t>(n);
+    int32_t* step_row = cv.take<int32_t>(n);
+    int32_t* nfall = cv.take<int32_t>(1);
     const int ni = static_cast<int>(n);
+    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
+    hipLaunchKernelGGL(row_topk, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0, s, P, ni, cval, cidx, rmax);
+    if (int rc = check_launch("row_topk")) return rc;
     if (method == 1) {
-        hipLaunchKernelGGL(row_max_kernel, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0, s, P, ni, rmax);
-        if (int rc = check_launch("row_max_kernel")) return rc;
         hipLaunchKernelGGL(stable_sort_desc, dim3(1), dim3(kSortThreads), 0, s, rmax, ni, ka, ia, kb, ib, order);
         if (int rc = check_launch("stable_sort_desc")) return rc;
     } else {
-        hipLaunchKernelGGL(identity_order, dim3(static_cast<unsigned>(div_up(n, 256))), dim3(256), 0, s, order, ni);
+        hipLaunchKernelGGL(identity_order, vgrid, dim3(256), 0, s, order, ni);
         if (int rc = check_launch("identity_order")) return rc;
     }
+    hipLaunchKernelGGL(fill_int, vgrid, dim3(256), 0, s, step_col, ni, -1);
+    if (int rc = check_launch("fill_int")) return rc;
     const size_t lds = sizeof(uint32_t) * static_cast<size_t>(div_up(n, 32));
     static bool attr_set = false;
     if (!attr_set && lds > 32768) {
@@ -217,8 +332,11 @@ int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* wo
                                   hipFuncAttributeMaxDynamicSharedMemorySize, v);
         attr_set = true;
     }
-    hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(kLapThreads), lds, s, P, ni, order, assignment);
-    return check_launch("greedy_walk");
+    hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(64), lds, s, P, ni, order, cval, cidx, assignment, step_col,
+                       step_row, nfall);
+    if (int rc = check_launch("greedy_walk")) return rc;
+    hipLaunchKernelGGL(mask_taken, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, P, ni, step_col, step_row);
+    return check_launch("mask_taken");
 }
 
 }  // extern "C"

@@ -78,6 +78,27 @@ def test_double_greedy_zero_row_takes_first_index(device):
     assert got == oracle.greedy_lap(M.clone(), "DOUBLE_GREEDY") == [0, 0, 0]
 
 
+@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY"])
+def test_greedy_fallback_paths_match_oracle(device, method):
+    # Identical rows: the row maxima tie (stable order) and after 16 rows every
+    # row's candidate list is taken, so the walk scans whole masked rows.  A
+    # zero block exercises the rule that a best value of 0 loses to the first
+    # (possibly taken) column.
+    n = 300
+    g = torch.Generator().manual_seed(8)
+    row = torch.rand(n, generator=g)
+    M = row.repeat(n, 1)
+    M[200:, :] = 0.0
+    M[250:, 5] = 1e-3
+    got = greedy_lap_device(M.clone().to(device), HungarianMethod[method])
+    Mr = M.clone()
+    want = oracle.greedy_lap(Mr, method)
+    assert got.cpu().tolist() == want
+    Md = M.clone().to(device)
+    greedy_lap_device(Md, HungarianMethod[method])
+    assert torch.equal(Md.cpu(), Mr)
+
+
 def test_greedy_full_size_is_permutation(device):
     n = 32768
     g = torch.Generator(device=device).manual_seed(0)
