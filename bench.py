@@ -98,8 +98,51 @@ def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: in
     }
 
 
+def probe(args) -> int:
+    """A few launches of every hot kernel on the bench workload, plus a
+    device copy of one n x n FP32 matrix (4 n^2 bytes read, 4 n^2 written)
+    that calibrates the FETCH_SIZE / WRITE_SIZE counters.  For profilers."""
+    import torch
+
+    from cugal_amd import _native as nat
+    from cugal_amd.adjacency import Adjacency
+    from cugal_amd.config import HungarianMethod
+    from cugal_amd.feature_extraction import Features, node_features
+    from cugal_amd.greedy_lap import greedy_lap_device
+    from cugal_amd.pred import _gradient_into, update_quasi_permutation_log
+    from cugal_amd.sinkhorn import LogKernel, solve_log_native
+    from cugal_amd.synthetic import watts_strogatz
+
+    n = args.n
+    inst = watts_strogatz(n, args.k, args.p, args.noise, seed=args.seed)
+    dev = torch.device("cuda:0")
+    A = Adjacency.from_graph(inst.source, dev)
+    B = Adjacency.from_graph(inst.target, dev)
+    feats = Features(node_features(A, args.mu), node_features(B, args.mu))
+    g = torch.Generator(device=dev).manual_seed(0)
+    P = torch.rand((n, n), generator=g, device=dev)
+    P /= P.sum(1, keepdim=True)
+    G = torch.empty_like(P)
+    for lam in (3, 4):
+        _gradient_into(G, A, B, P, feats, lam, 1.0)
+    u = torch.zeros(n, device=dev)
+    v = torch.zeros(n, device=dev)
+    solve_log_native(G, args.reg, u, v, 20, 0.0, 10, None, fused=True)
+    solve_log_native(G, args.reg, u, v, 10, 0.0, 10, None, fused=False)
+    gap = torch.empty(1, dtype=torch.float64, device=dev)
+    update_quasi_permutation_log(P, LogKernel(G, args.reg), u, v, 0.5, gap=gap)
+    greedy_lap_device(P, HungarianMethod.DOUBLE_GREEDY)
+    copy = torch.empty_like(P)
+    for _ in range(3):
+        copy.copy_(G)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "ok", "n": n, "copy_bytes": 4 * n * n}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--probe", action="store_true", help="launch each hot kernel a few times (for profilers)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -129,6 +172,8 @@ def main() -> int:
     from cugal_amd.synthetic import watts_strogatz
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm device"
+    if args.probe:
+        return probe(args)
     log(f"device {torch.cuda.get_device_name(0)}; generating WS(n={args.n}, k={args.k}, p={args.p})")
     t = time.perf_counter()
     inst = watts_strogatz(args.n, args.k, args.p, args.noise, seed=args.seed)

@@ -7,15 +7,14 @@
 // quantity; cugal/feature_extraction.py:98-108).
 //
 // Both sparse products are row gathers (out row r = sum of the input rows
-// listed by CSR row r), which read whole 256-byte row segments; the column
+// listed by CSR row r), which read whole 128-byte row segments; the column
 // side is turned into a row side by transposing through LDS:
-//   1. Y = (A P)^T : 64 x 64 tiles of A P gathered, stored transposed;
+//   1. Y = (A P)^T : 64 x 32 tiles of A P gathered, stored transposed;
 //   2. G = epilogue(-2 (B Y)^T): tiles of B Y gathered, transposed in LDS
 //      and written as rows of G with the distance and regulariser terms
 //      (B symmetric, so (B Y)^T = (A P) B).
 // Tiles are dealt so that the blocks one XCD runs together cover
-// consecutive row tiles of one column strip and share that XCD's L2 for
-// common neighbours.
+// consecutive row tiles of one 32-column strip, which fits that XCD's L2.
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -95,78 +94,105 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
 }
 
 // ---- tiled gather with a transposed store ---------------------------------
-// For the 64 x 64 tile (rows r0.., columns c0..) of Z = M X (M given by CSR):
+// For the 64 x 32 tile (rows r0.., columns c0..) of Z = M X (M given by CSR):
 //   Z[r][c] = sum_{k in N(r)} X[k][c],
 // write out[c][r] = Z[r][c] (EPI == false), or, with EPI, treat (c, r) as
 // (i, j) of the gradient: out[c][r] = scale * Z[r][c] + D(c, r) + rl (1 - 2 P[c][r]).
-constexpr int kTile = 64;
+// A 32-column strip of X is one 128-byte line per row, 4 MiB at n = 32768:
+// the L2 of one XCD.  Tiles are ordered so that an XCD sweeps all row tiles
+// of one strip before the next, so each line is fetched once and reused by
+// every row that lists it; outputs are stored non-temporally so they do not
+// evict the strip.
+constexpr int kTileR = 64;
+constexpr int kTileC = 32;
+
+__device__ __forceinline__ void st_stream4(float* p, float4 v) {
+    f32x4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(p));
+}
 
 template <bool EPI, bool VEC>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
-                                                           const float* __restrict__ X, float scale, Epilogue ep,
-                                                           float* __restrict__ out, int tiles_r, int64_t tiles) {
-    __shared__ float tile[kTile][kTile + 1];
+                                                           const float* __restrict__ X, int64_t ldx, float scale,
+                                                           Epilogue ep, float* __restrict__ out, int64_t ldo,
+                                                           int tiles_r, int64_t tiles) {
+    __shared__ float tile[kTileR][kTileC + 1];
     const int64_t t = xcd_tile(blockIdx.x, tiles);
     if (t >= tiles) return;
-    const int r0 = static_cast<int>(t % tiles_r) * kTile;
-    const int c0 = static_cast<int>(t / tiles_r) * kTile;
-    const int g = threadIdx.x >> 4;   // 16 groups of 16 threads
-    const int q = threadIdx.x & 15;   // float4 slot within a 64-float segment
-    const int cq = c0 + 4 * q;
-    for (int rr = g; rr < kTile; rr += 16) {
-        const int r = r0 + rr;
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < n) {
-            const int b = rp[r];
-            const int e = rp[r + 1];
-            if (VEC) {
-                if (cq < n) {
+    const int r0 = static_cast<int>(t % tiles_r) * kTileR;
+    const int c0 = static_cast<int>(t / tiles_r) * kTileC;
+    {
+        const int g = threadIdx.x >> 3;  // 32 row groups of 8 threads
+        const int q = threadIdx.x & 7;   // float4 slot within a 32-float line
+        const int cq = c0 + 4 * q;
+        for (int rr = g; rr < kTileR; rr += 32) {
+            const int r = r0 + rr;
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < n) {
+                const int b = rp[r];
+                const int e = rp[r + 1];
+                if (VEC) {
+                    if (cq < n) {
+                        for (int k = b; k < e; ++k) {
+                            const float4 x =
+                                *reinterpret_cast<const float4*>(X + static_cast<int64_t>(col[k]) * ldx + cq);
+                            acc.x += x.x;
+                            acc.y += x.y;
+                            acc.z += x.z;
+                            acc.w += x.w;
+                        }
+                    }
+                } else {
                     for (int k = b; k < e; ++k) {
-                        const float4 x = *reinterpret_cast<const float4*>(X + static_cast<int64_t>(col[k]) * n + cq);
-                        acc.x += x.x;
-                        acc.y += x.y;
-                        acc.z += x.z;
-                        acc.w += x.w;
+                        const float* xr = X + static_cast<int64_t>(col[k]) * ldx;
+                        if (cq + 0 < n) acc.x += xr[cq + 0];
+                        if (cq + 1 < n) acc.y += xr[cq + 1];
+                        if (cq + 2 < n) acc.z += xr[cq + 2];
+                        if (cq + 3 < n) acc.w += xr[cq + 3];
                     }
                 }
-            } else {
-                for (int k = b; k < e; ++k) {
-                    const float* xr = X + static_cast<int64_t>(col[k]) * n;
-                    if (cq + 0 < n) acc.x += xr[cq + 0];
-                    if (cq + 1 < n) acc.y += xr[cq + 1];
-                    if (cq + 2 < n) acc.z += xr[cq + 2];
-                    if (cq + 3 < n) acc.w += xr[cq + 3];
-                }
             }
+            tile[rr][4 * q + 0] = acc.x;
+            tile[rr][4 * q + 1] = acc.y;
+            tile[rr][4 * q + 2] = acc.z;
+            tile[rr][4 * q + 3] = acc.w;
         }
-        tile[rr][4 * q + 0] = acc.x;
-        tile[rr][4 * q + 1] = acc.y;
-        tile[rr][4 * q + 2] = acc.z;
-        tile[rr][4 * q + 3] = acc.w;
     }
     __syncthreads();
-    const int rq = r0 + 4 * q;  // four consecutive output columns
-    for (int cc = g; cc < kTile; cc += 16) {
+    const int g = threadIdx.x >> 4;  // 16 output rows per pass
+    const int q = threadIdx.x & 15;  // four consecutive output columns
+    const int rq = r0 + 4 * q;
+    for (int cc = g; cc < kTileC; cc += 16) {
         const int c = c0 + cc;
         if (c >= n) break;
         float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
-        const int64_t base = static_cast<int64_t>(c) * n;
+        const int64_t pbase = static_cast<int64_t>(c) * n;
+        const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
             const float4 fsi = ep.fs ? *reinterpret_cast<const float4*>(ep.fs + 4 * static_cast<int64_t>(c))
                                      : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (rq + e < n) v[e] = epi_one(v[e], scale, ep, base + rq + e, fsi, rq + e);
+                if (rq + e < n) v[e] = epi_one(v[e], scale, ep, pbase + rq + e, fsi, rq + e);
         }
         if (VEC && rq + 3 < n) {
-            *reinterpret_cast<float4*>(out + base + rq) = make_float4(v[0], v[1], v[2], v[3]);
+            st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 if (rq + e < n) out[base + rq + e] = v[e];
         }
     }
+}
+
+// out (row stride ldo) = X (row stride n): row-major streaming copy.
+__global__ __launch_bounds__(kSpThreads) void copy_rows_padded(const float* __restrict__ X, int n,
+                                                              float* __restrict__ out, int64_t ldo) {
+    const int64_t r = blockIdx.x;
+    const float* src = X + r * n;
+    float* dst = out + r * ldo;
+    for (int j = threadIdx.x * 4; j < n; j += kSpThreads * 4) st_stream4(dst + j, ld_stream4(src + j));
 }
 
 // ---- elementwise reference-compatible helpers ----------------------------
@@ -200,20 +226,21 @@ __global__ void regularize_kernel(float* __restrict__ g, const float* __restrict
 }
 
 template <bool EPI>
-int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, float scale, const Epilogue& ep,
-                       float* out, hipStream_t s) {
-    const bool vec = (n % 4 == 0) && aligned16(X) && aligned16(out) && (!ep.P || aligned16(ep.P)) &&
+int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
+                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
+    const bool vec = (n % 4 == 0) && (ldx % 4 == 0) && (ldo % 4 == 0) && aligned16(X) && aligned16(out) &&
+                     (!ep.P || aligned16(ep.P)) &&
                      (!ep.fs || (aligned16(ep.fs) && aligned16(ep.ft)));
-    const int tiles_r = static_cast<int>(div_up(n, kTile));
-    const int64_t tiles = static_cast<int64_t>(tiles_r) * tiles_r;
+    const int tiles_r = static_cast<int>(div_up(n, kTileR));
+    const int64_t tiles = static_cast<int64_t>(tiles_r) * div_up(n, kTileC);
     const int64_t blocks = div_up(tiles, kNumXcd) * kNumXcd;
     if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
     if (vec)
         hipLaunchKernelGGL((gather_tile_t<EPI, true>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
-                           col, n, X, scale, ep, out, tiles_r, tiles);
+                           col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     else
         hipLaunchKernelGGL((gather_tile_t<EPI, false>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
-                           rp, col, n, X, scale, ep, out, tiles_r, tiles);
+                           rp, col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     return check_launch("gather_tile_t");
 }
 
@@ -270,7 +297,15 @@ int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_
     return check_launch("regularize_kernel");
 }
 
-size_t cg_gradient_workspace(int64_t n) { return n > 0 ? sizeof(float) * static_cast<size_t>(n) * n + 256 : 0; }
+// Row stride of the intermediate: a power-of-two stride would map the lines
+// of one column strip onto the same few L2 sets.
+static int64_t gradient_ld(int64_t n) { return round_up(n, 4) + 64; }
+
+// Two padded n x n buffers: a copy of P (gathered by the first product) and
+// the intermediate Y.
+size_t cg_gradient_workspace(int64_t n) {
+    return n > 0 ? 2 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) : 0;
+}
 
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
                 int64_t n, const float* P, const float* source_features, const float* target_features, int lambda_,
@@ -285,13 +320,24 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     if (!workspace || workspace_bytes < cg_gradient_workspace(n))
         return fail(CG_EWORKSPACE, "cg_gradient: workspace too small");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    float* Y = static_cast<float*>(workspace);
     const int ni = static_cast<int>(n);
+    const int64_t ldy = gradient_ld(n);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    float* Pp = cv.take<float>(static_cast<size_t>(n) * ldy);
+    float* Y = cv.take<float>(static_cast<size_t>(n) * ldy);
     const Epilogue none{nullptr, nullptr, nullptr, 0.0f};
-    if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, P, 1.0f, none, Y, s)) return rc;
+    const float* X1 = P;
+    int64_t ld1 = n;
+    if (n % 4 == 0 && aligned16(P)) {
+        hipLaunchKernelGGL(copy_rows_padded, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, P, ni, Pp, ldy);
+        if (int rc = check_launch("copy_rows_padded")) return rc;
+        X1 = Pp;
+        ld1 = ldy;
+    }
+    if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
     Epilogue ep{rl != 0.0f ? P : nullptr, source_features, target_features, rl};
-    return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, -2.0f, ep, gradient, s);
+    return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
 }
 
 }  // extern "C"

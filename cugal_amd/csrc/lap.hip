@@ -12,11 +12,11 @@
 //   2. DOUBLE_GREEDY only: a stable descending sort of the row maxima (the
 //      reference's argsort is not stable: with tied maxima it may visit rows
 //      in another order);
-//   3. walk: one wave visits the rows in order.  The first candidate not yet
+//   3. walk: one block visits the rows in order.  The first candidate not yet
 //      taken is the argmax of the masked row whenever its value is > 0: every
 //      non-candidate is after it in (value desc, index asc) order and masked
 //      columns read 0.  Otherwise (all 16 taken, or a non-positive best) the
-//      wave scans the full masked row.  The next row's candidates are loaded
+//      block scans the full masked row.  The next row's candidates are loaded
 //      while the current one is decided.
 //   4. mask: P[r][j] *= 0 for every column j taken before row r was visited,
 //      which is what the in-place masking leaves in the matrix.
@@ -171,55 +171,93 @@ __global__ void identity_order(int32_t* __restrict__ order, int n) {
     if (i < n) order[i] = i;
 }
 
-// ---- 3. the sequential walk (one wave) -----------------------------------
+// ---- 3. the sequential walk ----------------------------------------------
+// Wave 0 decides each row from its candidates; the other waves join only the
+// rare full scans (all candidates taken, or a best value that is not > 0).
+constexpr int kWalkThreads = 256;
+
 __device__ __forceinline__ bool is_taken(const uint32_t* taken, int j) { return (taken[j >> 5] >> (j & 31)) & 1u; }
 
-__global__ __launch_bounds__(64) void greedy_walk(const float* __restrict__ P, int n, const int32_t* __restrict__ order,
-                                                 const float* __restrict__ cval, const int32_t* __restrict__ cidx,
-                                                 int32_t* __restrict__ assignment, int32_t* __restrict__ step_of_col,
-                                                 int32_t* __restrict__ step_of_row, int32_t* __restrict__ fallbacks) {
+__global__ __launch_bounds__(kWalkThreads) void greedy_walk(const float* __restrict__ P, int n,
+                                                           const int32_t* __restrict__ order,
+                                                           const float* __restrict__ cval,
+                                                           const int32_t* __restrict__ cidx,
+                                                           int32_t* __restrict__ assignment,
+                                                           int32_t* __restrict__ step_of_col,
+                                                           int32_t* __restrict__ step_of_row,
+                                                           int32_t* __restrict__ fallbacks) {
     extern __shared__ uint32_t taken[];  // n bits
-    const int lane = threadIdx.x;
+    __shared__ int s_choice;
+    __shared__ float wv[kWalkThreads / kWave];
+    __shared__ int wi[kWalkThreads / kWave];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
     const int words = static_cast<int>(div_up(n, 32));
-    for (int w = lane; w < words; w += 64) taken[w] = 0u;
+    for (int w = tid; w < words; w += kWalkThreads) taken[w] = 0u;
     __syncthreads();
     int nfall = 0;
-    int r = order[0];
-    float v = lane < kTopK ? cval[static_cast<int64_t>(r) * kTopK + lane] : -INFINITY;
-    int c = lane < kTopK ? cidx[static_cast<int64_t>(r) * kTopK + lane] : 0x7fffffff;
+    // Wave 0 keeps the current row's candidates in registers and loads the
+    // next row's while deciding.
+    float v = -INFINITY;
+    int c = 0x7fffffff;
+    if (wave == 0 && lane < kTopK) {
+        const int r0 = order[0];
+        v = cval[static_cast<int64_t>(r0) * kTopK + lane];
+        c = cidx[static_cast<int64_t>(r0) * kTopK + lane];
+    }
+    const bool vec = (n % 4) == 0;
     for (int k = 0; k < n; ++k) {
-        // Prefetch the next row's candidates.
-        int rn = 0;
-        float vn = -INFINITY;
-        int cn = 0x7fffffff;
-        if (k + 1 < n) {
-            rn = order[k + 1];
-            if (lane < kTopK) {
+        const int r = order[k];
+        if (wave == 0) {
+            float vn = -INFINITY;
+            int cn = 0x7fffffff;
+            if (k + 1 < n && lane < kTopK) {
+                const int rn = order[k + 1];
                 vn = cval[static_cast<int64_t>(rn) * kTopK + lane];
                 cn = cidx[static_cast<int64_t>(rn) * kTopK + lane];
             }
+            const bool free_c = lane < kTopK && c != 0x7fffffff && !is_taken(taken, c);
+            const unsigned long long m = __ballot(free_c);
+            int choice = -1;
+            if (m) {
+                const int first = __ffsll(m) - 1;
+                const float bv = __shfl(v, first, kWave);
+                const int bc = __shfl(c, first, kWave);
+                if (bv > 0.0f) choice = bc;
+            }
+            if (lane == 0) s_choice = choice;
+            v = vn;
+            c = cn;
         }
-        const bool free_c = lane < kTopK && c != 0x7fffffff && !is_taken(taken, c);
-        const unsigned long long m = __ballot(free_c);
-        int choice = -1;
-        if (m) {
-            const int first = __ffsll(m) - 1;
-            const float bv = __shfl(v, first, kWave);
-            const int bc = __shfl(c, first, kWave);
-            if (bv > 0.0f) choice = bc;
-        }
+        __syncthreads();
+        int choice = s_choice;
         if (choice < 0) {
-            // Full scan of the masked row: taken columns read x * 0.
-            ++nfall;
+            // Full scan of the masked row by the whole block: taken columns
+            // read x * 0; first index of the maximum.
             const float* row = P + static_cast<int64_t>(r) * n;
             float bv = -INFINITY;
             int bi = 0x7fffffff;
-            for (int j = lane; j < n; j += 64) {
-                float x = row[j];
-                if (is_taken(taken, j)) x = x * 0.0f;
-                if (before(x, j, bv, bi)) {
-                    bv = x;
-                    bi = j;
+            if (vec) {
+                for (int j = tid * 4; j < n; j += kWalkThreads * 4) {
+                    const float4 x4 = *reinterpret_cast<const float4*>(row + j);
+                    const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float x = is_taken(taken, j + e) ? xs[e] * 0.0f : xs[e];
+                        if (before(x, j + e, bv, bi)) {
+                            bv = x;
+                            bi = j + e;
+                        }
+                    }
+                }
+            } else {
+                for (int j = tid; j < n; j += kWalkThreads) {
+                    const float x = is_taken(taken, j) ? row[j] * 0.0f : row[j];
+                    if (before(x, j, bv, bi)) {
+                        bv = x;
+                        bi = j;
+                    }
                 }
             }
 #pragma unroll
@@ -231,20 +269,34 @@ __global__ __launch_bounds__(64) void greedy_walk(const float* __restrict__ P, i
                     bi = oi;
                 }
             }
-            choice = bi == 0x7fffffff ? 0 : bi;  // an all-NaN row: index 0
+            if (lane == 0) {
+                wv[wave] = bv;
+                wi[wave] = bi;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                float fv = wv[0];
+                int fi = wi[0];
+                for (int w = 1; w < kWalkThreads / kWave; ++w)
+                    if (before(wv[w], wi[w], fv, fi)) {
+                        fv = wv[w];
+                        fi = wi[w];
+                    }
+                s_choice = fi == 0x7fffffff ? 0 : fi;  // an all-NaN row: index 0
+            }
+            __syncthreads();
+            choice = s_choice;
+            ++nfall;
         }
-        if (lane == 0) {
+        if (tid == 0) {
             assignment[r] = choice;
             step_of_row[r] = k;
             if (step_of_col[choice] < 0) step_of_col[choice] = k;
             taken[choice >> 5] |= 1u << (choice & 31);
         }
         __syncthreads();
-        r = rn;
-        v = vn;
-        c = cn;
     }
-    if (lane == 0 && fallbacks) *fallbacks = nfall;
+    if (tid == 0 && fallbacks) *fallbacks = nfall;
 }
 
 __global__ void fill_int(int32_t* __restrict__ p, int n, int value) {
@@ -332,8 +384,8 @@ int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* wo
                                   hipFuncAttributeMaxDynamicSharedMemorySize, v);
         attr_set = true;
     }
-    hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(64), lds, s, P, ni, order, cval, cidx, assignment, step_col,
-                       step_row, nfall);
+    hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(kWalkThreads), lds, s, P, ni, order, cval, cidx, assignment,
+                       step_col, step_row, nfall);
     if (int rc = check_launch("greedy_walk")) return rc;
     hipLaunchKernelGGL(mask_taken, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, P, ni, step_col, step_row);
     return check_launch("mask_taken");
