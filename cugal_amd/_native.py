@@ -54,6 +54,16 @@ _SIGNATURES = {
         c_int,
         [c_p, c_i64, c_f, c_p, c_p, c_int, c_d, c_int, c_int, c_int, c_p, c_int, c_p, c_p, c_p, c_p, c_p, c_sz, c_p],
     ),
+    "cg_feature_cost": (c_int, [c_p, c_p, c_p, c_f, c_i64, c_p, c_p]),
+    "cg_sinkhorn_plan": (c_int, [c_p, c_i64, c_f, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    "cg_sinkhorn_checkpoint_floats": (c_sz, [c_i64, c_int, c_int]),
+    "cg_sinkhorn_unrolled_workspace": (c_sz, [c_i64, c_int]),
+    "cg_sinkhorn_unrolled_forward": (
+        c_int, [c_p, c_i64, c_f, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p],
+    ),
+    "cg_sinkhorn_unrolled_backward": (
+        c_int, [c_p, c_i64, c_f, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_f, c_p, c_p, c_sz, c_p],
+    ),
     "cg_update_workspace": (c_sz, [c_i64]),
     "cg_update_quasi_permutation_log": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
     "cg_lap_workspace": (c_sz, [c_i64]),

@@ -657,7 +657,75 @@ int timer_total(float* ms, int* launches) {
     return 0;
 }
 
+__global__ void sk_ctl_reset(SinkCtl* __restrict__ ctl) {
+    ctl->done = 0;
+    ctl->iterations = 0;
+    ctl->final_buf = 0;
+    ctl->nerr = 0;
+    ctl->ticket = 0u;
+    ctl->momentum = 1.0f;
+}
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Unrolled segment (used by the differentiable Sinkhorn): `steps` plain
+// iterations from a checkpoint, recording every potential in base-2 units.
+// u_hist[0] = u_c, v_hist[0] = v_c (copied); u_hist[s], v_hist[s] are the
+// potentials after iteration s.  The first v-update is the exact column
+// sweep; later ones come from the fused sweep when it applies, so a segment
+// recomputed from the same checkpoint reproduces the same bits.
+// ---------------------------------------------------------------------------
+size_t sinkhorn_segment_workspace(int64_t n) {
+    const Plan p = make_plan(n, cg_sinkhorn_fused_supported(n) && n % 4 == 0);
+    Carver cv{nullptr, 0, ~size_t(0)};
+    return carve(cv, n, p, 4, nullptr) + 256;
+}
+
+int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, const float* v_c, int steps,
+                     float* u_hist, float* v_hist, void* ws, size_t ws_bytes, hipStream_t s) {
+    const int use_fused = cg_sinkhorn_fused_supported(n) && n % 4 == 0 && aligned16(C);
+    const Plan p = make_plan(n, use_fused);
+    Carver cv{static_cast<char*>(ws), 0, ws_bytes};
+    SinkWs w{};
+    carve(cv, n, p, 4, &w);
+    if (!ws || !cv.ok()) return fail(CG_EWORKSPACE, "sinkhorn segment: workspace too small");
+    const int64_t nn = n;
+    CG_CHECK_HIP(hipMemcpyAsync(u_hist, u_c, sizeof(float) * nn, hipMemcpyDeviceToDevice, s));
+    CG_CHECK_HIP(hipMemcpyAsync(v_hist, v_c, sizeof(float) * nn, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(sk_ctl_reset, dim3(1), dim3(1), 0, s, w.ctl);
+    if (int rc = check_launch("sk_ctl_reset")) return rc;
+    auto comb = [&](bool pairs, const float* vin, float* vout) -> int {
+        if (pairs)
+            hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+                               p.col_nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl);
+        else
+            hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+                               p.nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl);
+        return check_launch("sk_combine");
+    };
+    if (steps <= 0) return CG_OK;
+    if (int rc = launch_col(C, n, n, scale2, v_hist, u_hist, static_cast<float2*>(w.part), p, w.ctl, s)) return rc;
+    if (int rc = comb(true, v_hist, v_hist + nn)) return rc;
+    for (int q = 1; q <= steps; ++q) {
+        float* uq = u_hist + q * nn;
+        float* vq = v_hist + q * nn;
+        if (use_fused) {
+            if (int rc = launch_fused(C, n, scale2, vq, uq, static_cast<float*>(w.part), p, w.ctl, s)) return rc;
+            if (q < steps)
+                if (int rc = comb(false, vq, vq + nn)) return rc;
+        } else {
+            if (int rc = launch_row(C, n, n, scale2, vq, 1.0f, 1.0f, uq, 1.0f, w.ctl, s)) return rc;
+            if (q < steps) {
+                if (int rc = launch_col(C, n, n, scale2, vq, uq, static_cast<float2*>(w.part), p, w.ctl, s))
+                    return rc;
+                if (int rc = comb(true, vq, vq + nn)) return rc;
+            }
+        }
+    }
+    return CG_OK;
+}
+
 }  // namespace cg
 
 using namespace cg;

@@ -165,6 +165,41 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
                           int* sweep_launches, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 /* ------------------------------------------------------------------------
+ * Differentiable unrolled log-Sinkhorn.  No reference counterpart: the
+ * reference's only differentiable form is torch.logsumexp in its CPU branch
+ * (cugal/sinkhorn.py:107-111), whose iteration this adjoint follows.
+ * ---------------------------------------------------------------------- */
+
+/* C (n x n) = lam * sum_k weights[k] |source[i][k] - target[j][k]| (weighted
+ * L1 over the 4 features); weights is a device array of 4 floats. */
+int cg_feature_cost(const float* source_features, const float* target_features, const float* weights, float lam,
+                    int64_t n, float* cost, cg_stream_t stream);
+
+/* plan = exp(-C / reg + u_i + v_j) (natural-log duals). Workspace >= 8n + 512 bytes. */
+int cg_sinkhorn_plan(const float* C, int64_t n, float regularization, const float* u, const float* v, float* plan,
+                     void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
+/* Forward: `iterations` plain log-domain iterations (v then u, no stopping
+ * test) from the duals u0, v0; writes the final duals u, v and opaque
+ * checkpoints (cg_sinkhorn_checkpoint_floats() floats) holding the duals
+ * every `checkpoint_every` iterations (1..64).  Backward: given
+ * grad_plan = dL/dplan (plan = exp(-C/reg + u + v) of the final duals),
+ * recomputes each checkpoint segment and writes dL/dC into grad_cost
+ * (nullable) and, when source_features is given (C built by
+ * cg_feature_cost with `weights`, `lam`), grad_lam_w[0] = dL/dlam and
+ * grad_lam_w[1..4] = dL/dweights (device doubles). */
+size_t cg_sinkhorn_checkpoint_floats(int64_t n, int iterations, int checkpoint_every);
+size_t cg_sinkhorn_unrolled_workspace(int64_t n, int checkpoint_every);
+int cg_sinkhorn_unrolled_forward(const float* C, int64_t n, float regularization, int iterations,
+                                 int checkpoint_every, const float* u0, const float* v0, float* checkpoints,
+                                 float* u, float* v, void* workspace, size_t workspace_bytes, cg_stream_t stream);
+int cg_sinkhorn_unrolled_backward(const float* C, int64_t n, float regularization, int iterations,
+                                  int checkpoint_every, const float* checkpoints, const float* grad_plan,
+                                  float* grad_cost, const float* source_features, const float* target_features,
+                                  const float* weights, float lam, double* grad_lam_w, void* workspace,
+                                  size_t workspace_bytes, cg_stream_t stream);
+
+/* ------------------------------------------------------------------------
  * Frank-Wolfe update.
  * ---------------------------------------------------------------------- */
 

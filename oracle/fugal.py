@@ -350,3 +350,28 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
         col = list(scipy_lap(Q))
     mapping = [(i, int(col[i])) for i in range(n)]
     return OracleRun(Q, mapping, iterations, gaps)
+
+
+# ---------------------------------------------------------------------------
+# Differentiable references (BASELINE differentiation form)
+# ---------------------------------------------------------------------------
+
+def unrolled_plan(C: torch.Tensor, reg: float, iterations: int, u0=None, v0=None) -> torch.Tensor:
+    """`iterations` plain log-domain iterations of cugal/sinkhorn.py:107-111
+    (momentum 1, no stopping test) from (u0, v0) = 0 by default; returns
+    exp(K + u + v).  Differentiable with torch autograd; any device.
+    Pin: the separable-cost and 2x2 closed forms via solve_log, whose loop
+    this is, and FP64 central differences of its own loss."""
+    n = C.shape[0]
+    K = -C / reg
+    u = torch.zeros(n, dtype=C.dtype, device=C.device) if u0 is None else u0
+    v = torch.zeros(n, dtype=C.dtype, device=C.device) if v0 is None else v0
+    for _ in range(iterations):
+        v = -torch.logsumexp(K + u[:, None], 0)
+        u = -torch.logsumexp(K + v[None, :], 1)
+    return torch.exp(K + u[:, None] + v[None, :])
+
+
+def feature_cost(fs: torch.Tensor, ft: torch.Tensor, w: torch.Tensor, lam) -> torch.Tensor:
+    """lam * sum_k w_k |fs_ik - ft_jk| (weighted L1 over the features)."""
+    return lam * (w[None, None, :] * (fs[:, None, :] - ft[None, :, :]).abs()).sum(-1)

@@ -289,3 +289,17 @@ def test_pipeline_default_config_small():
     inst = erdos_renyi(60, 0.1, 0.0, seed=2)
     run = oracle.align(inst.source, inst.target, oracle.OracleConfig(iter_count=4, frank_wolfe_iter_count=4))
     assert alignment_accuracy(inst.source_mapping, answer_columns(run.mapping)) >= 0.9
+
+
+def test_unrolled_plan_is_solve_log_and_feature_cost_brute_force():
+    n = 30
+    C = torch.rand((n, n), dtype=torch.float64, generator=torch.Generator().manual_seed(4)) * 3
+    st = oracle.SinkhornState(n, "LOG")
+    K, u, v, _ = st.solve_log(C, 0.4, 17, 0.0, 10)
+    np.testing.assert_allclose(oracle.unrolled_plan(C, 0.4, 17).numpy(), oracle.plan_log(K, u, v).numpy(), rtol=1e-13)
+    fs = torch.rand((5, 4), dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    ft = torch.rand((5, 4), dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    w = torch.tensor([0.5, 1.0, 1.5, 2.0], dtype=torch.float64)
+    want = np.array([[0.3 * sum(float(w[k]) * abs(float(fs[i, k] - ft[j, k])) for k in range(4)) for j in range(5)]
+                     for i in range(5)])
+    np.testing.assert_allclose(oracle.feature_cost(fs, ft, w, 0.3).numpy(), want, rtol=1e-14)
