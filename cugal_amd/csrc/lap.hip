@@ -1,25 +1,32 @@
 // Greedy rounding of the quasi-permutation to a permutation,
-// cugal/greedy_lap.py:7-63 (GREEDY :10-19, DOUBLE_GREEDY :50-63).
+// cugal/greedy_lap.py:7-85 (GREEDY :10-19, RAND :21-32, MORE_RAND :34-48,
+// DOUBLE_GREEDY :50-63, PARALLEL_GREEDY :65-85).
 //
-// Reference semantics: rows are visited in order (GREEDY: 0..n-1;
-// DOUBLE_GREEDY: by descending row maximum); each visited row r is
-// multiplied in place by the 0/1 "not taken" vector and the first index of
-// its maximum is taken.
+// Reference semantics: rows are visited in order (GREEDY: 0..n-1; RAND and
+// MORE_RAND: a random permutation; DOUBLE_GREEDY: by descending row maximum;
+// PARALLEL_GREEDY: rows whose maximum is > 0.5 take their argmax at once, the
+// rest are visited by descending maximum); each visited row r is multiplied
+// in place by the 0/1 "not taken" vector and the first index of its maximum
+// is taken (MORE_RAND: the first index whose prefix sum reaches total * U).
 //
 // MI355X form (exact, not a heuristic):
 //   1. row_topk: every row's 16 largest entries ordered by (value desc,
 //      index asc), one wave per row, fully parallel;
-//   2. DOUBLE_GREEDY only: a stable descending sort of the row maxima (the
-//      reference's argsort is not stable: with tied maxima it may visit rows
-//      in another order);
+//   2. DOUBLE_GREEDY / PARALLEL_GREEDY: a stable descending sort of the row
+//      maxima (the reference's sorts are not stable: with tied maxima they
+//      may visit rows in another order); PARALLEL_GREEDY then splits the
+//      sorted rows into the parallel set and the walk order in one block;
 //   3. walk: one block visits the rows in order.  The first candidate not yet
 //      taken is the argmax of the masked row whenever its value is > 0: every
 //      non-candidate is after it in (value desc, index asc) order and masked
 //      columns read 0.  Otherwise (all 16 taken, or a non-positive best) the
 //      block scans the full masked row.  The next row's candidates are loaded
-//      while the current one is decided.
+//      while the current one is decided.  MORE_RAND walks with a block-wide
+//      FP64 prefix sum of each masked row instead;
 //   4. mask: P[r][j] *= 0 for every column j taken before row r was visited,
-//      which is what the in-place masking leaves in the matrix.
+//      which is what the in-place masking leaves in the matrix.  Steps count
+//      from 1; columns taken by PARALLEL_GREEDY's parallel set carry step 0,
+//      and its parallel rows (step 0) are never masked.
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -185,7 +192,9 @@ __global__ __launch_bounds__(kWalkThreads) void greedy_walk(const float* __restr
                                                            int32_t* __restrict__ assignment,
                                                            int32_t* __restrict__ step_of_col,
                                                            int32_t* __restrict__ step_of_row,
-                                                           int32_t* __restrict__ fallbacks) {
+                                                           int32_t* __restrict__ fallbacks,
+                                                           const int32_t* __restrict__ walk_count,
+                                                           const uint32_t* __restrict__ taken_init) {
     extern __shared__ uint32_t taken[];  // n bits
     __shared__ int s_choice;
     __shared__ float wv[kWalkThreads / kWave];
@@ -194,25 +203,26 @@ __global__ __launch_bounds__(kWalkThreads) void greedy_walk(const float* __restr
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
     const int words = static_cast<int>(div_up(n, 32));
-    for (int w = tid; w < words; w += kWalkThreads) taken[w] = 0u;
+    for (int w = tid; w < words; w += kWalkThreads) taken[w] = taken_init ? taken_init[w] : 0u;
+    const int count = walk_count ? *walk_count : n;
     __syncthreads();
     int nfall = 0;
     // Wave 0 keeps the current row's candidates in registers and loads the
     // next row's while deciding.
     float v = -INFINITY;
     int c = 0x7fffffff;
-    if (wave == 0 && lane < kTopK) {
+    if (wave == 0 && lane < kTopK && count > 0) {
         const int r0 = order[0];
         v = cval[static_cast<int64_t>(r0) * kTopK + lane];
         c = cidx[static_cast<int64_t>(r0) * kTopK + lane];
     }
     const bool vec = (n % 4) == 0;
-    for (int k = 0; k < n; ++k) {
+    for (int k = 0; k < count; ++k) {
         const int r = order[k];
         if (wave == 0) {
             float vn = -INFINITY;
             int cn = 0x7fffffff;
-            if (k + 1 < n && lane < kTopK) {
+            if (k + 1 < count && lane < kTopK) {
                 const int rn = order[k + 1];
                 vn = cval[static_cast<int64_t>(rn) * kTopK + lane];
                 cn = cidx[static_cast<int64_t>(rn) * kTopK + lane];
@@ -290,13 +300,187 @@ __global__ __launch_bounds__(kWalkThreads) void greedy_walk(const float* __restr
         }
         if (tid == 0) {
             assignment[r] = choice;
-            step_of_row[r] = k;
-            if (step_of_col[choice] < 0) step_of_col[choice] = k;
+            step_of_row[r] = k + 1;
+            if (step_of_col[choice] < 0) step_of_col[choice] = k + 1;
             taken[choice >> 5] |= 1u << (choice & 31);
         }
         __syncthreads();
     }
     if (tid == 0 && fallbacks) *fallbacks = nfall;
+}
+
+// ---- PARALLEL_GREEDY: the parallel set and the walk order ----------------
+// One block.  Rows of the sorted order whose maximum is > 0.5 take their
+// first-index argmax (the head of their candidate list) and mark it taken;
+// the others keep their sorted order in walk_order (a stable compaction).
+// NaN maxima are not > 0.5, so those rows are walked, as in the reference.
+__global__ __launch_bounds__(kSortThreads) void split_parallel(const int32_t* __restrict__ order, int n,
+                                                              const float* __restrict__ rmax,
+                                                              const int32_t* __restrict__ cidx,
+                                                              int32_t* __restrict__ assignment,
+                                                              int32_t* __restrict__ step_of_col,
+                                                              int32_t* __restrict__ step_of_row,
+                                                              uint32_t* __restrict__ taken_init,
+                                                              int32_t* __restrict__ walk_order,
+                                                              int32_t* __restrict__ walk_count) {
+    __shared__ int before_me[kSortThreads];
+    const int words = static_cast<int>(div_up(n, 32));
+    for (int w = threadIdx.x; w < words; w += kSortThreads) taken_init[w] = 0u;
+    const int per = static_cast<int>(div_up(n, kSortThreads));
+    const int b = threadIdx.x * per;
+    const int e = min(n, b + per);
+    int walked = 0;
+    for (int k = b; k < e; ++k) walked += !(rmax[order[k]] > 0.5f);
+    before_me[threadIdx.x] = walked;
+    __syncthreads();  // also orders the bitmap clear before the atomics below
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < kSortThreads; ++t) {
+            const int v = before_me[t];
+            before_me[t] = run;
+            run += v;
+        }
+        *walk_count = run;
+    }
+    __syncthreads();
+    int pos = before_me[threadIdx.x];
+    for (int k = b; k < e; ++k) {
+        const int r = order[k];
+        if (rmax[r] > 0.5f) {
+            const int c = cidx[static_cast<int64_t>(r) * kTopK];
+            assignment[r] = c;
+            step_of_row[r] = 0;
+            step_of_col[c] = 0;
+            atomicOr(&taken_init[c >> 5], 1u << (c & 31));
+        } else {
+            walk_order[pos++] = r;
+        }
+    }
+}
+
+// ---- RAND / MORE_RAND input check ----------------------------------------
+// order must be a permutation of 0..n-1 and every draw must lie in [0, 1].
+__global__ __launch_bounds__(kSortThreads) void check_random_inputs(const int32_t* __restrict__ order, int n,
+                                                                   const double* __restrict__ uniforms,
+                                                                   uint32_t* __restrict__ seen,
+                                                                   int32_t* __restrict__ bad) {
+    const int words = static_cast<int>(div_up(n, 32));
+    for (int w = threadIdx.x; w < words; w += kSortThreads) seen[w] = 0u;
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    int my_bad = 0;
+    for (int k = threadIdx.x; k < n; k += kSortThreads) {
+        const int r = order[k];
+        if (r < 0 || r >= n) {
+            my_bad = 1;
+            continue;
+        }
+        const uint32_t bit = 1u << (r & 31);
+        if (atomicOr(&seen[r >> 5], bit) & bit) my_bad = 1;
+        if (uniforms) {
+            const double u = uniforms[k];
+            if (!(u >= 0.0 && u <= 1.0)) my_bad = 1;
+        }
+    }
+    if (my_bad) atomicOr(&s_bad, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *bad = s_bad;
+}
+
+// ---- MORE_RAND: sampling walk --------------------------------------------
+// One block of 1024.  For the k-th visited row r: thread t owns a contiguous
+// slice of columns and runs the FP64 prefix sum of the masked row over it,
+// starting from the block-wide exclusive prefix of the slice sums; that
+// running sum is the row's cumsum.  r_k = total * U_k, and the column is the
+// first j with r_k <= cumsum_j: the first slice whose running maximum
+// reaches r_k holds it, and its owner rescans the slice to find it.  No such
+// column (only possible with negative entries, or a NaN) writes -1.
+constexpr int kSampleThreads = 1024;
+
+__global__ __launch_bounds__(kSampleThreads) void sample_walk(const float* __restrict__ P, int n,
+                                                             const int32_t* __restrict__ order,
+                                                             const double* __restrict__ uniforms,
+                                                             int32_t* __restrict__ assignment,
+                                                             int32_t* __restrict__ step_of_col,
+                                                             int32_t* __restrict__ step_of_row) {
+    extern __shared__ uint32_t taken[];  // n bits
+    __shared__ double wtot[kSampleThreads / kWave];
+    __shared__ double s_total;
+    __shared__ int s_first;
+    __shared__ int s_choice;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int words = static_cast<int>(div_up(n, 32));
+    for (int w = tid; w < words; w += kSampleThreads) taken[w] = 0u;
+    const int per = static_cast<int>(div_up(n, kSampleThreads));
+    const int b = min(n, tid * per);
+    const int e = min(n, b + per);
+    const int last_owner = (n - 1) / per;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const int r = order[k];
+        const float* row = P + static_cast<int64_t>(r) * n;
+        double acc = 0.0;
+        for (int j = b; j < e; ++j) {
+            const float x = is_taken(taken, j) ? row[j] * 0.0f : row[j];
+            acc += static_cast<double>(x);
+        }
+        // Block exclusive prefix of the slice sums.
+        double incl = acc;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double o = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += o;
+        }
+        double excl = __shfl_up(incl, 1, kWave);
+        if (lane == 0) excl = 0.0;
+        if (lane == kWave - 1) wtot[wave] = incl;
+        if (tid == 0) s_first = 0x7fffffff;
+        __syncthreads();
+        double wbase = 0.0;
+        for (int w = 0; w < wave; ++w) wbase += wtot[w];
+        excl += wbase;
+        // Running sums of the slice: the end value (the total, for the
+        // owner of column n-1) and the maximum.
+        double run = excl, top = -INFINITY;
+        for (int j = b; j < e; ++j) {
+            const float x = is_taken(taken, j) ? row[j] * 0.0f : row[j];
+            run += static_cast<double>(x);
+            top = fmax(top, run);
+        }
+        if (tid == last_owner) s_total = run;
+        __syncthreads();
+        const double target = s_total * uniforms[k];
+        if (b < e && target <= top) atomicMin(&s_first, tid);
+        __syncthreads();
+        const int first = s_first;
+        if (tid == first) {
+            int choice = -1;
+            double rs = excl;
+            for (int j = b; j < e; ++j) {
+                const float x = is_taken(taken, j) ? row[j] * 0.0f : row[j];
+                rs += static_cast<double>(x);
+                if (target <= rs) {
+                    choice = j;
+                    break;
+                }
+            }
+            s_choice = choice;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int choice = first == 0x7fffffff ? -1 : s_choice;
+            assignment[r] = choice;
+            step_of_row[r] = k + 1;
+            if (choice >= 0) {
+                if (step_of_col[choice] < 0) step_of_col[choice] = k + 1;
+                taken[choice >> 5] |= 1u << (choice & 31);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void fill_int(int32_t* __restrict__ p, int n, int value) {
@@ -322,73 +506,157 @@ __global__ __launch_bounds__(256) void mask_taken(float* __restrict__ P, int n,
 
 using namespace cg;
 
+namespace {
+
+struct LapWs {
+    float* cval;
+    int32_t* cidx;
+    float* rmax;
+    uint32_t* ka;
+    int32_t* ia;
+    uint32_t* kb;
+    int32_t* ib;
+    int32_t* order;
+    int32_t* step_col;
+    int32_t* step_row;
+    int32_t* nfall;
+    int32_t* walk_order;
+    uint32_t* taken_init;
+    int32_t* walk_count;
+    uint32_t* seen;
+    int32_t* bad;
+};
+
+LapWs lap_carve(char* base, int64_t n, size_t bytes, size_t* used = nullptr) {
+    Carver cv{base, 0, bytes};
+    const int64_t words = div_up(n, 32);
+    LapWs w;
+    w.cval = cv.take<float>(n * kTopK);
+    w.cidx = cv.take<int32_t>(n * kTopK);
+    w.rmax = cv.take<float>(n);
+    w.ka = cv.take<uint32_t>(n);
+    w.ia = cv.take<int32_t>(n);
+    w.kb = cv.take<uint32_t>(n);
+    w.ib = cv.take<int32_t>(n);
+    w.order = cv.take<int32_t>(n);
+    w.step_col = cv.take<int32_t>(n);
+    w.step_row = cv.take<int32_t>(n);
+    w.nfall = cv.take<int32_t>(1);
+    w.walk_order = cv.take<int32_t>(n);
+    w.taken_init = cv.take<uint32_t>(words);
+    w.walk_count = cv.take<int32_t>(1);
+    w.seen = cv.take<uint32_t>(words);
+    w.bad = cv.take<int32_t>(1);
+    if (used) *used = cv.used;
+    return w;
+}
+
+// Raise the dynamic LDS limit of a walk kernel once, when its n-bit taken
+// bitmap needs more than the default.
+void allow_lds(const void* kernel, size_t lds, bool& done) {
+    if (done || lds <= 32768) return;
+    int dev = 0, v = 65536;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    done = true;
+}
+
+enum class Walk { kGreedy, kDouble, kParallel, kRand, kMoreRand };
+
+int lap_run(float* P, int64_t n, Walk method, const int32_t* order_in, const double* uniforms, int32_t* assignment,
+            void* workspace, size_t workspace_bytes, hipStream_t s, const char* who) {
+    if (n > kLapMaxN)
+        return fail(CG_EUNSUPPORTED, "%s: n=%lld exceeds %lld", who, static_cast<long long>(n),
+                    static_cast<long long>(kLapMaxN));
+    if (!workspace || workspace_bytes < cg_lap_workspace(n)) return fail(CG_EWORKSPACE, "%s: workspace too small", who);
+    LapWs w = lap_carve(static_cast<char*>(workspace), n, workspace_bytes);
+    const int ni = static_cast<int>(n);
+    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
+    const bool random = method == Walk::kRand || method == Walk::kMoreRand;
+    if (random) {
+        // A bad order would index out of bounds: check it before any walk.
+        hipLaunchKernelGGL(check_random_inputs, dim3(1), dim3(kSortThreads), 0, s, order_in, ni, uniforms, w.seen,
+                           w.bad);
+        if (int rc = check_launch("check_random_inputs")) return rc;
+        int bad = 1;
+        CG_CHECK_HIP(hipMemcpyAsync(&bad, w.bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        CG_CHECK_HIP(hipStreamSynchronize(s));
+        if (bad)
+            return fail(CG_EINVAL, "%s: order is not a permutation of 0..n-1 or a draw is outside [0, 1]", who);
+    }
+    const int32_t* order = random ? order_in : w.order;
+    if (method != Walk::kMoreRand) {
+        hipLaunchKernelGGL(row_topk, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0, s, P, ni, w.cval, w.cidx,
+                           w.rmax);
+        if (int rc = check_launch("row_topk")) return rc;
+    }
+    if (method == Walk::kDouble || method == Walk::kParallel) {
+        hipLaunchKernelGGL(stable_sort_desc, dim3(1), dim3(kSortThreads), 0, s, w.rmax, ni, w.ka, w.ia, w.kb, w.ib,
+                           w.order);
+        if (int rc = check_launch("stable_sort_desc")) return rc;
+    } else if (method == Walk::kGreedy) {
+        hipLaunchKernelGGL(identity_order, vgrid, dim3(256), 0, s, w.order, ni);
+        if (int rc = check_launch("identity_order")) return rc;
+    }
+    hipLaunchKernelGGL(fill_int, vgrid, dim3(256), 0, s, w.step_col, ni, -1);
+    if (int rc = check_launch("fill_int")) return rc;
+    const size_t lds = sizeof(uint32_t) * static_cast<size_t>(div_up(n, 32));
+    if (method == Walk::kMoreRand) {
+        static bool sample_attr = false;
+        allow_lds(reinterpret_cast<const void*>(&sample_walk), lds, sample_attr);
+        hipLaunchKernelGGL(sample_walk, dim3(1), dim3(kSampleThreads), lds, s, P, ni, order, uniforms, assignment,
+                           w.step_col, w.step_row);
+        if (int rc = check_launch("sample_walk")) return rc;
+    } else {
+        const int32_t* walk_count = nullptr;
+        const uint32_t* taken_init = nullptr;
+        if (method == Walk::kParallel) {
+            hipLaunchKernelGGL(split_parallel, dim3(1), dim3(kSortThreads), 0, s, w.order, ni, w.rmax, w.cidx,
+                               assignment, w.step_col, w.step_row, w.taken_init, w.walk_order, w.walk_count);
+            if (int rc = check_launch("split_parallel")) return rc;
+            order = w.walk_order;
+            walk_count = w.walk_count;
+            taken_init = w.taken_init;
+        }
+        static bool walk_attr = false;
+        allow_lds(reinterpret_cast<const void*>(&greedy_walk), lds, walk_attr);
+        hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(kWalkThreads), lds, s, P, ni, order, w.cval, w.cidx, assignment,
+                           w.step_col, w.step_row, w.nfall, walk_count, taken_init);
+        if (int rc = check_launch("greedy_walk")) return rc;
+    }
+    hipLaunchKernelGGL(mask_taken, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, P, ni, w.step_col, w.step_row);
+    return check_launch("mask_taken");
+}
+
+}  // namespace
+
 extern "C" {
 
 size_t cg_lap_workspace(int64_t n) {
     if (n <= 0) return 0;
-    Carver cv{nullptr, 0, ~size_t(0)};
-    cv.take<float>(n * kTopK);
-    cv.take<int32_t>(n * kTopK);
-    cv.take<float>(n);
-    cv.take<uint32_t>(n);
-    cv.take<int32_t>(n);
-    cv.take<uint32_t>(n);
-    cv.take<int32_t>(n);
-    cv.take<int32_t>(n);
-    cv.take<int32_t>(n);
-    cv.take<int32_t>(n);
-    cv.take<int32_t>(1);
-    return cv.used + 256;
+    size_t used = 0;
+    (void)lap_carve(nullptr, n, ~size_t(0), &used);
+    return used + 256;
 }
 
 int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* workspace, size_t workspace_bytes,
                   cg_stream_t stream) {
     if (!P || !assignment || n <= 0) return fail(CG_EINVAL, "cg_lap_greedy: bad arguments");
-    if (n > kLapMaxN) return fail(CG_EUNSUPPORTED, "cg_lap_greedy: n=%lld exceeds %lld", static_cast<long long>(n),
-                                  static_cast<long long>(kLapMaxN));
-    if (method != 0 && method != 1) return fail(CG_EINVAL, "cg_lap_greedy: unknown method %d", method);
-    if (!workspace || workspace_bytes < cg_lap_workspace(n)) return fail(CG_EWORKSPACE, "cg_lap_greedy: workspace too small");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
-    float* cval = cv.take<float>(n * kTopK);
-    int32_t* cidx = cv.take<int32_t>(n * kTopK);
-    float* rmax = cv.take<float>(n);
-    uint32_t* ka = cv.take<uint32_t>(n);
-    int32_t* ia = cv.take<int32_t>(n);
-    uint32_t* kb = cv.take<uint32_t>(n);
-    int32_t* ib = cv.take<int32_t>(n);
-    int32_t* order = cv.take<int32_t>(n);
-    int32_t* step_col = cv.take<int32_t>(n);
-    int32_t* step_row = cv.take<int32_t>(n);
-    int32_t* nfall = cv.take<int32_t>(1);
-    const int ni = static_cast<int>(n);
-    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
-    hipLaunchKernelGGL(row_topk, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0, s, P, ni, cval, cidx, rmax);
-    if (int rc = check_launch("row_topk")) return rc;
-    if (method == 1) {
-        hipLaunchKernelGGL(stable_sort_desc, dim3(1), dim3(kSortThreads), 0, s, rmax, ni, ka, ia, kb, ib, order);
-        if (int rc = check_launch("stable_sort_desc")) return rc;
-    } else {
-        hipLaunchKernelGGL(identity_order, vgrid, dim3(256), 0, s, order, ni);
-        if (int rc = check_launch("identity_order")) return rc;
-    }
-    hipLaunchKernelGGL(fill_int, vgrid, dim3(256), 0, s, step_col, ni, -1);
-    if (int rc = check_launch("fill_int")) return rc;
-    const size_t lds = sizeof(uint32_t) * static_cast<size_t>(div_up(n, 32));
-    static bool attr_set = false;
-    if (!attr_set && lds > 32768) {
-        int dev = 0, v = 65536;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&greedy_walk),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, v);
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(kWalkThreads), lds, s, P, ni, order, cval, cidx, assignment,
-                       step_col, step_row, nfall);
-    if (int rc = check_launch("greedy_walk")) return rc;
-    hipLaunchKernelGGL(mask_taken, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, P, ni, step_col, step_row);
-    return check_launch("mask_taken");
+    if (method < 0 || method > 2) return fail(CG_EINVAL, "cg_lap_greedy: unknown method %d", method);
+    const Walk walk = method == 0 ? Walk::kGreedy : method == 1 ? Walk::kDouble : Walk::kParallel;
+    return lap_run(P, n, walk, nullptr, nullptr, assignment, workspace, workspace_bytes,
+                   reinterpret_cast<hipStream_t>(stream), "cg_lap_greedy");
+}
+
+int cg_lap_greedy_random(float* P, int64_t n, int method, const int32_t* order, const double* uniforms,
+                         int32_t* assignment, void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    if (!P || !assignment || !order || n <= 0) return fail(CG_EINVAL, "cg_lap_greedy_random: bad arguments");
+    if (method != 0 && method != 1) return fail(CG_EINVAL, "cg_lap_greedy_random: unknown method %d", method);
+    if (method == 1 && !uniforms) return fail(CG_EINVAL, "cg_lap_greedy_random: MORE_RAND needs the draws");
+    return lap_run(P, n, method == 0 ? Walk::kRand : Walk::kMoreRand, order, method == 1 ? uniforms : nullptr,
+                   assignment, workspace, workspace_bytes, reinterpret_cast<hipStream_t>(stream),
+                   "cg_lap_greedy_random");
 }
 
 }  // extern "C"

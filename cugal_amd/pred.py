@@ -184,7 +184,8 @@ def hungarian(quasi_permutation: torch.Tensor, config: Config, profile: Profile)
     match config.hungarian_method:
         case HungarianMethod.SCIPY:
             _, column_indices = scipy.optimize.linear_sum_assignment(quasi_permutation.cpu().numpy(), maximize=True)
-        case HungarianMethod.GREEDY | HungarianMethod.DOUBLE_GREEDY:
+        case (HungarianMethod.GREEDY | HungarianMethod.RAND | HungarianMethod.MORE_RAND | HungarianMethod.DOUBLE_GREEDY
+              | HungarianMethod.PARALLEL_GREEDY):
             column_indices = greedy_lap(quasi_permutation, config)
         case HungarianMethod.SPARSE:
             sparse = scipy.sparse.csr_matrix(quasi_permutation.cpu().numpy())

@@ -218,16 +218,31 @@ int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const f
  * Rounding to a permutation (LAP).
  * ---------------------------------------------------------------------- */
 
-/* Replaces greedy_lap (cugal/greedy_lap.py:7-63) for
+/* Replaces greedy_lap (cugal/greedy_lap.py:7-85) for
  *   method 0 = GREEDY (rows in index order, :10-19),
  *   method 1 = DOUBLE_GREEDY (rows by descending row maximum, :50-63; ties
- *              in row order, i.e. a stable sort).
+ *              in row order, i.e. a stable sort),
+ *   method 2 = PARALLEL_GREEDY (:65-85: rows whose maximum is > 0.5 take the
+ *              first index of it directly and are not masked; the others are
+ *              visited by descending maximum with those columns taken).
  * For each row in order, the columns already taken are multiplied by 0 in P
  * (in place, as the reference does) and the first index of the row maximum
  * is taken.  assignment (device, n int32) receives the column of each row. */
 size_t cg_lap_workspace(int64_t n);
 int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* workspace, size_t workspace_bytes,
                   cg_stream_t stream);
+
+/* Replaces the RAND (cugal/greedy_lap.py:21-32) and MORE_RAND (:34-48)
+ * branches: method 0 = RAND, 1 = MORE_RAND.  order (device, n int32) is the
+ * row order the reference draws with torch.randperm; it must be a
+ * permutation of 0..n-1 (checked, with a stream synchronisation).  MORE_RAND
+ * also takes uniforms (device, n doubles in [0, 1]): the k-th visited row's
+ * random.random() draw U_k, so that r_k = total_k * U_k stands for the
+ * reference's uniform(0, cumsum[-1]); the column is the first j whose FP64
+ * prefix sum of the masked row is >= r_k (-1 if none, which needs negative
+ * entries or a NaN).  Same workspace and in-place masking as cg_lap_greedy. */
+int cg_lap_greedy_random(float* P, int64_t n, int method, const int32_t* order, const double* uniforms,
+                         int32_t* assignment, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -239,24 +239,56 @@ def duality_gap(gradient, step, alpha) -> float:
 # LAP
 # ---------------------------------------------------------------------------
 
-def greedy_lap(cost_matrix: torch.Tensor, method: str) -> list[int]:
-    """GREEDY (cugal/greedy_lap.py:10-19) and DOUBLE_GREEDY (:50-63), in
-    place on cost_matrix like the reference, with a stable row order.
-    Pin: brute force over permutations where greedy is optimal; hand-built
-    cases of the masking rule."""
+def greedy_lap(cost_matrix: torch.Tensor, method: str, order=None, uniforms=None) -> list[int]:
+    """GREEDY (cugal/greedy_lap.py:10-19), RAND (:21-32), MORE_RAND
+    (:34-48), DOUBLE_GREEDY (:50-63) and PARALLEL_GREEDY (:65-85), in place
+    on cost_matrix like the reference, with stable sorts.
+
+    RAND and MORE_RAND take the row order the reference draws with
+    torch.randperm (`order`); MORE_RAND also takes the k-th visited row's
+    draw U_k of random.random() (`uniforms`): the reference's
+    uniform(0, total) is 0 + total * U_k.  Its prefix sums are taken in FP64
+    (the reference's device cumsum rounds in FP32 in a parallel order that no
+    restatement reproduces); the column is the first j with r <= cumsum_j.
+
+    Pins: brute force over permutations where greedy is optimal; RAND with
+    the identity / the sorted order equals GREEDY / DOUBLE_GREEDY; hand-built
+    masking cases; PARALLEL_GREEDY's two regimes (all maxima above 0.5: plain
+    argmax; none: DOUBLE_GREEDY); MORE_RAND's selection frequencies against
+    the masked row's proportions."""
     n = cost_matrix.size(1)
     taken = torch.ones(n, dtype=cost_matrix.dtype)
     res = [0] * n
     if method == "GREEDY":
         order = range(cost_matrix.size(0))
+    elif method in ("RAND", "MORE_RAND"):
+        order = [int(r) for r in order]
+        if sorted(order) != list(range(cost_matrix.size(0))):
+            raise ValueError("order must be a permutation of the rows")
     elif method == "DOUBLE_GREEDY":
         max_values_in_rows = cost_matrix.max(dim=1).values
         order = torch.sort(max_values_in_rows, descending=True, stable=True).indices.tolist()
+    elif method == "PARALLEL_GREEDY":
+        max_values_in_rows, max_col_in_rows = cost_matrix.max(dim=1)
+        order_values, order_t = torch.sort(max_values_in_rows, descending=True, stable=True)
+        half_mask = order_values > 0.5
+        for row in order_t[half_mask].tolist():
+            res[row] = int(max_col_in_rows[row])
+            taken[res[row]] = 0
+        order = order_t[~half_mask].tolist()
     else:
         raise ValueError(method)
-    for row in order:
+    for k, row in enumerate(order):
         cost_matrix[row] *= taken
-        m = int(cost_matrix[row].argmax())
+        if method == "MORE_RAND":
+            cumsum = np.cumsum(cost_matrix[row].to(torch.float64).numpy())
+            r = cumsum[-1] * float(uniforms[k])
+            hits = np.nonzero(r <= cumsum)[0]
+            if len(hits) == 0:
+                raise IndexError("no prefix reaches the draw")
+            m = int(hits[0])
+        else:
+            m = int(cost_matrix[row].argmax())
         taken[m] = 0
         res[row] = m
     return res

@@ -1,10 +1,11 @@
 """FW update (cugal/pred.py:77-86, :133-141) and greedy LAP
-(cugal/greedy_lap.py:10-63) against the oracle."""
+(cugal/greedy_lap.py:10-85) against the oracle."""
 
 import numpy as np
 import pytest
 import torch
 
+from cugal_amd._native import NativeError
 from cugal_amd.config import HungarianMethod
 from cugal_amd.greedy_lap import greedy_lap_device
 from cugal_amd.pred import update_quasi_permutation_log
@@ -56,17 +57,74 @@ def _distinct_rowmax_matrix(n, seed):
     return M
 
 
-@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY"])
+ALL_GREEDY = ["GREEDY", "DOUBLE_GREEDY", "PARALLEL_GREEDY", "RAND", "MORE_RAND"]
+
+
+def _run_both(device, M, method, seed):
+    """The device method and the oracle on the same matrix and draws;
+    returns (device assignment, oracle assignment, device P, oracle P)."""
+    n = M.shape[0]
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(n)
+    uniforms = rng.random(n)
+    Md = M.clone().to(device)
+    got = greedy_lap_device(Md, HungarianMethod[method], order=torch.from_numpy(order),
+                            uniforms=torch.from_numpy(uniforms)).cpu().tolist()
+    Mr = M.clone()
+    want = oracle.greedy_lap(Mr, method, order=order.tolist(), uniforms=uniforms.tolist())
+    return got, want, Md.cpu(), Mr
+
+
+@pytest.mark.parametrize("method", ALL_GREEDY)
 @pytest.mark.parametrize("n", [50, 1000, 1537])
 def test_greedy_matches_oracle(device, method, n):
     M = _distinct_rowmax_matrix(n, seed=n)
-    Md = M.to(device)
-    got = greedy_lap_device(Md, HungarianMethod[method]).cpu().numpy()
-    Mr = M.clone()
-    want = oracle.greedy_lap(Mr, method)
-    assert got.tolist() == want
+    got, want, Md, Mr = _run_both(device, M, method, seed=n)
+    assert got == want
     # the matrix is masked in place exactly like the reference's
-    assert torch.equal(Md.cpu(), Mr)
+    assert torch.equal(Md, Mr)
+
+
+def test_parallel_greedy_shared_column(device):
+    # Two rows above 0.5 on the same column both take it (the reference
+    # assigns the parallel set without checking), and neither is masked.
+    M = torch.tensor([[0.6, 0.4, 0.0], [0.7, 0.3, 0.0], [0.2, 0.5, 0.3]])
+    got, want, Md, Mr = _run_both(device, M, "PARALLEL_GREEDY", seed=0)
+    assert got == want == [0, 0, 1]
+    assert torch.equal(Md, Mr) and Md[0, 0] == 0.6 and Md[2, 0] == 0.0
+
+
+def test_random_methods_reject_bad_inputs(device):
+    M = torch.rand((64, 64), device=device)
+    with pytest.raises(NativeError):
+        greedy_lap_device(M, HungarianMethod.RAND, order=torch.zeros(64, dtype=torch.int64))
+    with pytest.raises(NativeError):
+        greedy_lap_device(M, HungarianMethod.RAND, order=torch.arange(1, 65))
+    with pytest.raises(NativeError):
+        greedy_lap_device(M, HungarianMethod.MORE_RAND, order=torch.arange(64),
+                          uniforms=torch.full((64,), 1.5, dtype=torch.float64))
+
+
+def test_more_rand_unreachable_draw_raises(device):
+    # Prefix sums [-1, -0.5] never reach 0.1 * (-0.5): the reference's
+    # mask.nonzero()[0] raises, and so do both restatements.
+    M = torch.tensor([[-1.0, 0.5], [0.2, 0.8]])
+    with pytest.raises(IndexError):
+        greedy_lap_device(M.to(device), HungarianMethod.MORE_RAND, order=torch.tensor([0, 1]),
+                          uniforms=torch.tensor([0.1, 0.5], dtype=torch.float64))
+    with pytest.raises(IndexError):
+        oracle.greedy_lap(M.clone(), "MORE_RAND", order=[0, 1], uniforms=[0.1, 0.5])
+
+
+def test_random_methods_draw_like_the_reference(device):
+    # With no order given, RAND visits rows in torch.randperm(n, device)
+    # order: same seed, same order, same result as passing that order.
+    M = _distinct_rowmax_matrix(500, seed=4).to(device)
+    torch.manual_seed(11)
+    got = greedy_lap_device(M.clone(), HungarianMethod.RAND)
+    torch.manual_seed(11)
+    order = torch.randperm(500, device=device)
+    assert torch.equal(got, greedy_lap_device(M.clone(), HungarianMethod.RAND, order=order))
 
 
 def test_double_greedy_zero_row_takes_first_index(device):
@@ -78,7 +136,7 @@ def test_double_greedy_zero_row_takes_first_index(device):
     assert got == oracle.greedy_lap(M.clone(), "DOUBLE_GREEDY") == [0, 0, 0]
 
 
-@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY"])
+@pytest.mark.parametrize("method", ALL_GREEDY)
 def test_greedy_fallback_paths_match_oracle(device, method):
     # Identical rows: the row maxima tie (stable order) and after 16 rows every
     # row's candidate list is taken, so the walk scans whole masked rows.  A
@@ -90,13 +148,9 @@ def test_greedy_fallback_paths_match_oracle(device, method):
     M = row.repeat(n, 1)
     M[200:, :] = 0.0
     M[250:, 5] = 1e-3
-    got = greedy_lap_device(M.clone().to(device), HungarianMethod[method])
-    Mr = M.clone()
-    want = oracle.greedy_lap(Mr, method)
-    assert got.cpu().tolist() == want
-    Md = M.clone().to(device)
-    greedy_lap_device(Md, HungarianMethod[method])
-    assert torch.equal(Md.cpu(), Mr)
+    got, want, Md, Mr = _run_both(device, M, method, seed=8)
+    assert got == want
+    assert torch.equal(Md, Mr)
 
 
 def test_greedy_full_size_is_permutation(device):
@@ -107,3 +161,11 @@ def test_greedy_full_size_is_permutation(device):
     M[torch.arange(n, device=device), perm] += 1.0
     got = greedy_lap_device(M, HungarianMethod.DOUBLE_GREEDY)
     assert torch.equal(got.long(), perm)
+
+
+@pytest.mark.parametrize("method", ["PARALLEL_GREEDY", "RAND", "MORE_RAND"])
+def test_greedy_variants_at_8192_match_oracle(device, method):
+    M = _distinct_rowmax_matrix(8192, seed=5)
+    got, want, Md, Mr = _run_both(device, M, method, seed=5)
+    assert got == want
+    assert torch.equal(Md, Mr)

@@ -247,17 +247,80 @@ def test_scipy_lap_brute_force(seed):
     assert np.array_equal(oracle.scipy_lap(M), _brute_lap(M.numpy()))
 
 
-@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY"])
+@pytest.mark.parametrize("method", ["GREEDY", "DOUBLE_GREEDY", "RAND", "PARALLEL_GREEDY"])
 def test_greedy_optimal_on_dominant_matrix(method):
     # With one entry per row larger than the rest of its row and column,
-    # greedy is optimal, so it equals the brute-force LAP.
+    # greedy is optimal in any row order, so it equals the brute-force LAP.
     n = 7
     g = torch.Generator().manual_seed(1)
     perm = torch.randperm(n, generator=g)
     M = torch.rand((n, n), generator=g, dtype=torch.float64) * 0.1
     M[torch.arange(n), perm] = 1.0 + torch.rand(n, generator=g, dtype=torch.float64)
     want = _brute_lap(M.numpy())
-    assert oracle.greedy_lap(M.clone(), method) == list(want)
+    order = torch.randperm(n, generator=g).tolist()
+    assert oracle.greedy_lap(M.clone(), method, order=order) == list(want)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rand_with_fixed_orders_is_greedy_and_double_greedy(seed):
+    g = torch.Generator().manual_seed(seed)
+    M = torch.rand((9, 9), generator=g, dtype=torch.float64)
+    assert oracle.greedy_lap(M.clone(), "RAND", order=list(range(9))) == oracle.greedy_lap(M.clone(), "GREEDY")
+    sorted_rows = sorted(range(9), key=lambda r: -float(M[r].max()))
+    assert oracle.greedy_lap(M.clone(), "RAND", order=sorted_rows) == oracle.greedy_lap(M.clone(), "DOUBLE_GREEDY")
+
+
+def test_parallel_greedy_regimes():
+    g = torch.Generator().manual_seed(3)
+    # No row maximum above 0.5: the walk alone, i.e. DOUBLE_GREEDY.
+    M = torch.rand((8, 8), generator=g, dtype=torch.float64) * 0.5
+    assert oracle.greedy_lap(M.clone(), "PARALLEL_GREEDY") == oracle.greedy_lap(M.clone(), "DOUBLE_GREEDY")
+    # Every maximum above 0.5 on distinct columns: the plain row argmax, and
+    # P is left unmasked.
+    perm = torch.randperm(8, generator=g)
+    M = torch.rand((8, 8), generator=g, dtype=torch.float64) * 0.5
+    M[torch.arange(8), perm] = 0.6
+    Q = M.clone()
+    assert oracle.greedy_lap(Q, "PARALLEL_GREEDY") == perm.tolist()
+    assert torch.equal(Q, M)
+    # Mixed: row 0 (0.9 at column 0) is served in parallel; row 1, whose
+    # maximum is column 0 as well, is walked with column 0 masked.
+    M = torch.tensor([[0.9, 0.05, 0.05], [0.45, 0.4, 0.15], [0.1, 0.2, 0.3]], dtype=torch.float64)
+    Q = M.clone()
+    assert oracle.greedy_lap(Q, "PARALLEL_GREEDY") == [0, 1, 2]
+    assert Q[1, 0] == 0 and Q[0, 0] == 0.9 and Q[2, 0] == 0
+
+
+def test_more_rand_inverse_cdf_and_masking():
+    M = torch.tensor([[0.2, 0.5, 0.3], [0.6, 0.3, 0.1], [0.0, 0.0, 1.0]], dtype=torch.float64)
+    # Row 0 first: U in (0, 0.2] -> 0, (0.2, 0.7] -> 1, (0.7, 1) -> 2.  Row 1
+    # then has the taken column masked to 0.
+    for u0, want0 in ((0.1, 0), (0.3, 1), (0.69, 1), (0.71, 2), (0.999, 2)):
+        Q = M.clone()
+        res = oracle.greedy_lap(Q, "MORE_RAND", order=[0, 1, 2], uniforms=[u0, 0.5, 0.5])
+        assert res[0] == want0
+        assert Q[1, want0] == 0
+    # A draw of 0 takes column 0 even where the entry is 0 (r <= cumsum).
+    Q = M.clone()
+    assert oracle.greedy_lap(Q, "MORE_RAND", order=[2, 0, 1], uniforms=[0.0, 0.5, 0.5])[2] == 0
+    # A permutation matrix: every positive draw takes its one column.
+    perm = [2, 0, 1]
+    Pm = torch.zeros((3, 3), dtype=torch.float64)
+    Pm[[0, 1, 2], perm] = 1.0
+    assert oracle.greedy_lap(Pm, "MORE_RAND", order=[1, 2, 0], uniforms=[0.3, 0.9, 1e-9]) == perm
+
+
+def test_more_rand_frequencies():
+    # The first visited row is sampled in proportion to its entries.
+    rng = np.random.default_rng(0)
+    M = torch.tensor([[0.1, 0.4, 0.2, 0.3], [1.0, 0, 0, 0], [0, 1.0, 0, 0], [0, 0, 1.0, 0]], dtype=torch.float64)
+    draws = 4000
+    counts = np.zeros(4)
+    for u in rng.random(draws):
+        counts[oracle.greedy_lap(M.clone(), "MORE_RAND", order=[0, 1, 2, 3], uniforms=[u, 0.5, 0.5, 0.5])[0]] += 1
+    p = np.array([0.1, 0.4, 0.2, 0.3])
+    sigma = np.sqrt(draws * p * (1 - p))
+    assert np.all(np.abs(counts - draws * p) < 4 * sigma)
 
 
 def test_double_greedy_order_and_masking():
