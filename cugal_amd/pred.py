@@ -24,6 +24,7 @@ from .adjacency import Adjacency
 from .config import Config, HungarianMethod, SinkhornMethod
 from .feature_extraction import Features
 from .greedy_lap import greedy_lap
+from .hungarian import dense_hungarian
 from .profile import Phase, Profile, SinkhornProfile, TimeStamp
 from .sinkhorn import LogKernel, SinkhornState
 
@@ -187,6 +188,8 @@ def hungarian(quasi_permutation: torch.Tensor, config: Config, profile: Profile)
         case (HungarianMethod.GREEDY | HungarianMethod.RAND | HungarianMethod.MORE_RAND | HungarianMethod.DOUBLE_GREEDY
               | HungarianMethod.PARALLEL_GREEDY):
             column_indices = greedy_lap(quasi_permutation, config)
+        case HungarianMethod.DENSE:
+            column_indices = dense_hungarian(quasi_permutation, maximize=True).cpu().numpy().astype(np.int64)
         case HungarianMethod.SPARSE:
             sparse = scipy.sparse.csr_matrix(quasi_permutation.cpu().numpy())
             _, column_indices = scipy.sparse.csgraph.min_weight_full_bipartite_matching(sparse, maximize=True)

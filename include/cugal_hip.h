@@ -244,6 +244,22 @@ int cg_lap_greedy(float* P, int64_t n, int method, int32_t* assignment, void* wo
 int cg_lap_greedy_random(float* P, int64_t n, int method, const int32_t* order, const double* uniforms,
                          int32_t* assignment, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
+/* Replaces dense_hungarian (cuda_kernels/hungarian/hungarian.cu:436-448,
+ * bound at cuda_kernels/kernels.cpp:69) and the cugraph dense Hungarian of
+ * HungarianMethod.DENSE (cugal/pred.py:161-166): an exact linear assignment
+ * of the n x n FP32 matrix `cost` (row-major, not modified), minimising the
+ * sum (maximize = 0) or maximising it (maximize = 1; the reference passes
+ * 1 - P to a minimiser, which rounds small entries of P away in FP32; here
+ * P is maximised directly).  Shortest augmenting paths with FP64 duals,
+ * warm-started from column and row reductions.  assignment (device, n
+ * int32) receives the column of each row; stats (device, 2 int32, may be
+ * null) receives the number of rows left free by the warm start and the
+ * total Dijkstra steps.  n <= 32768.  Synchronises the stream (it reports
+ * NaN or infinite costs as CG_EINVAL). */
+size_t cg_lap_hungarian_workspace(int64_t n);
+int cg_lap_hungarian(const float* cost, int64_t n, int maximize, int32_t* assignment, int32_t* stats,
+                     void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
