@@ -8,7 +8,7 @@ the row-gather SpMM kernel; there is no host or torch fallback.
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import networkx as nx
 import numpy as np
@@ -53,6 +53,8 @@ class Adjacency:
     row_pointers: torch.Tensor
     """Offset in col_indices where each row begins; length n + 1 (int32)."""
 
+    _dense_bf16: torch.Tensor | None = field(default=None, init=False, repr=False, compare=False)
+
     @classmethod
     def from_dense(cls, dense: torch.Tensor) -> "Adjacency":
         """From a dense (n x n) matrix on the device; non-zeros become ones."""
@@ -84,6 +86,20 @@ class Adjacency:
         dense = torch.zeros((n, n), dtype=dtype, device=self.col_indices.device)
         dense[rows, self.col_indices.to(torch.int64)] = 1
         return dense
+
+    def dense_bf16(self) -> torch.Tensor:
+        """The 0/1 adjacency as a dense BF16 matrix (exact), zero padded to
+        n rounded up to 256 in both dimensions: the operand of the MFMA
+        gradient.  Built once on the device and cached on the object."""
+        if self._dense_bf16 is None:
+            n = self.number_of_nodes()
+            dev = self.col_indices.device
+            elems = int(nat.query("cg_dense_adjacency_elements", n))
+            out = torch.empty(elems, dtype=torch.bfloat16, device=dev)
+            nat.call("cg_adjacency_densify_bf16", nat.ptr(self.col_indices), nat.ptr(self.row_pointers), n,
+                     nat.ptr(out), nat.stream_of(dev))
+            self._dense_bf16 = out
+        return self._dense_bf16
 
     def byte_size(self) -> int:
         return 4 * len(self.col_indices) + 4 * len(self.row_pointers)

@@ -121,8 +121,10 @@ class Config:
     one pass, n <= 32768) instead of twice.  Same iterates either way."""
 
     gradient_precision: str = "fp32"
-    """Arithmetic of the A P B contraction: "fp32" (sparse adjacency times
-    dense FP32 P)."""
+    """Arithmetic of the A P B contraction: "fp32" (CSR adjacency times
+    dense FP32 P) or "bf16x2" (dense 0/1 BF16 adjacency times P split into
+    BF16 hi + lo on the matrix cores, FP32 accumulation; pays off for dense
+    adjacencies)."""
 
     def convert_tensor(self, input: torch.Tensor) -> torch.Tensor:
         """Convert tensor to correct type and dtype."""

@@ -1,0 +1,338 @@
+// Frank-Wolfe gradient on the matrix cores (the precision form):
+//   G = -2 A P B + D + f * lambda * (1 - 2 P)
+// with A, B the 0/1 adjacencies of undirected graphs as dense BF16 (exact)
+// and P split into BF16 hi + lo (hi = bf16(P), lo = bf16(P - hi)), every
+// product accumulated in FP32.  Same quantity as cugal/pred.py:38-51
+// (dense, f = 2) and :54-74 (sparse, f = 1); the reference runs it as torch
+// matmuls / cuSPARSE SpMM in FP32.
+//
+// Three launches over matrices padded with zeros to np = n rounded up to
+// 256 (so every tile is whole):
+//   1. split_transpose: P^T -> PT_hi, PT_lo (BF16, np x np), through LDS;
+//   2. gemm_nt<split R>: Y = A P, i.e. Y[m][c] = sum_k A[m][k] PT[c][k]
+//      (hi and lo products into the same accumulators), written back as
+//      Y_hi, Y_lo;
+//   3. gemm_nt<split L>: G[m][c] = -2 sum_k Y[m][k] B[c][k] (= -2 Y B, B
+//      symmetric) with the distance (from the 4-feature rows) and the
+//      regulariser fused in the epilogue.
+// Both GEMMs are "NT": each operand row holds consecutive k, so an MFMA
+// fragment is one 16-byte read.  A block computes a 256 x 256 tile with four
+// waves of 128 x 128 (v_mfma_f32_32x32x16_bf16, 4 x 4 tiles of 32 x 32),
+// staging 32-wide k slices of the three input tiles (single operand, hi, lo)
+// in two LDS buffers with an 80-byte row stride (conflict-free 16-byte
+// reads).  Tiles are ordered in groups of 4 tile rows and dealt so that the
+// 32 blocks one XCD runs together share 4 row panels and 8 column panels
+// in its L2.
+#include "common.hpp"
+#include "cugal_hip.h"
+
+namespace cg {
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+constexpr int kTile = 256;      // block tile (rows and columns)
+constexpr int kBK = 32;         // k slice per LDS stage
+constexpr int kThreads = 256;   // four waves
+constexpr int kRowB = 80;       // LDS bytes per tile row: 64 + 16 padding
+constexpr int kTileB = kTile * kRowB;           // one operand tile in LDS
+constexpr int kStageB = 3 * kTileB;             // single + hi + lo
+constexpr int kLdsB = 2 * kStageB;              // two stages: 120 KiB
+constexpr int kGroup = 4;       // tile rows per group in the block order
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ uint16_t bf16_bits(__bf16 x) { return __builtin_bit_cast(uint16_t, x); }
+
+__device__ __forceinline__ void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+    const __bf16 h = static_cast<__bf16>(x);
+    const __bf16 l = static_cast<__bf16>(x - static_cast<float>(h));
+    hi = bf16_bits(h);
+    lo = bf16_bits(l);
+}
+
+// ---- dense 0/1 BF16 adjacency from CSR ----------------------------------
+__global__ __launch_bounds__(256) void densify_rows(const int32_t* __restrict__ col, const int32_t* __restrict__ rp,
+                                                   int n, int64_t np, uint16_t* __restrict__ out) {
+    const int64_t i = blockIdx.x;
+    const int b = rp[i];
+    const int e = rp[i + 1];
+    for (int k = b + static_cast<int>(threadIdx.x); k < e; k += 256) {
+        const int j = col[k];
+        if (j >= 0 && j < n) out[i * np + j] = 0x3F80;  // 1.0
+    }
+}
+
+// ---- 1. PT_hi / PT_lo = split(P^T), zero padded --------------------------
+__global__ __launch_bounds__(256) void split_transpose(const float* __restrict__ P, int n, int64_t np,
+                                                      uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+    __shared__ float tile[64][65];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * 64;  // rows of P
+    const int64_t c0 = static_cast<int64_t>(blockIdx.x) * 64;  // columns of P
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t r = r0 + i, c = c0 + tx;
+        tile[i][tx] = (r < n && c < n) ? P[r * n + c] : 0.0f;
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        uint16_t h, l;
+        split_bf16(tile[tx][j], h, l);  // P[r0 + tx][c0 + j]
+        const int64_t o = (c0 + j) * np + r0 + tx;
+        hi[o] = h;
+        lo[o] = l;
+    }
+}
+
+struct GradEpi {
+    const float* P;   // nullptr: no regulariser term
+    const float* fs;  // source features n x 4 (nullptr: no distance term)
+    const float* ft;
+    float rl;         // f * lambda
+    float* G;         // n x n
+    uint16_t* y_hi;   // gemm 1 output (np x np)
+    uint16_t* y_lo;
+};
+
+// The grid has kNumXcd * ceil(tiles / kNumXcd) blocks; returns false for
+// the surplus ones.
+__device__ __forceinline__ bool tile_of_block(int64_t bid, int tiles_1d, int& tm, int& tn) {
+    const int64_t tiles = static_cast<int64_t>(tiles_1d) * tiles_1d;
+    const int64_t per = div_up(tiles, kNumXcd);
+    const int64_t t = (bid % kNumXcd) * per + bid / kNumXcd;  // an XCD's blocks: consecutive t
+    if (t >= tiles) return false;
+    const int64_t in_group = static_cast<int64_t>(kGroup) * tiles_1d;
+    const int64_t g = t / in_group;
+    const int64_t first = g * kGroup;
+    const int rows = static_cast<int>(min<int64_t>(kGroup, tiles_1d - first));
+    const int64_t w = t % in_group;
+    tm = static_cast<int>(first + w % rows);
+    tn = static_cast<int>(w / rows);
+    return true;
+}
+
+// C[m][c] = sum_k L[m][k] R[c][k] over k < np, with the split operand on the
+// left (SPLIT_L: L = Lh + Ll, R single) or on the right (R = Rh + Rl).
+// EPI 0: write C as Y_hi / Y_lo; EPI 1: gradient epilogue into G.
+template <bool SPLIT_L, int EPI>
+__global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__ X,   // the single operand
+                                                   const uint16_t* __restrict__ Sh,  // split operand, hi
+                                                   const uint16_t* __restrict__ Sl,  // split operand, lo
+                                                   int n, int64_t np, int tiles_1d, float scale, GradEpi ep) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    int tm, tn;
+    if (!tile_of_block(blockIdx.x, tiles_1d, tm, tn)) return;
+    const int64_t m0 = static_cast<int64_t>(tm) * kTile;
+    const int64_t c0 = static_cast<int64_t>(tn) * kTile;
+    // Rows of the three tiles: the single operand's rows and the split
+    // operand's rows (L rows are m, R rows are c).
+    const int64_t x_row0 = SPLIT_L ? c0 : m0;
+    const int64_t s_row0 = SPLIT_L ? m0 : c0;
+
+    // Global -> LDS: per stage, each tile is 256 rows x 4 chunks of 16 B;
+    // thread t moves chunks t, t + 256, t + 512, t + 768 of each.
+    u32x4 stage_x[4], stage_h[4], stage_l[4];
+    auto load_global = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int q = tid + s * kThreads;
+            const int row = q >> 2, ch = q & 3;
+            const int64_t kx = k0 + ch * 8;
+            stage_x[s] = *reinterpret_cast<const u32x4*>(X + (x_row0 + row) * np + kx);
+            stage_h[s] = *reinterpret_cast<const u32x4*>(Sh + (s_row0 + row) * np + kx);
+            stage_l[s] = *reinterpret_cast<const u32x4*>(Sl + (s_row0 + row) * np + kx);
+        }
+    };
+    auto store_lds = [&](int stage) {
+        char* base = lds + stage * kStageB;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int q = tid + s * kThreads;
+            const int row = q >> 2, ch = q & 3;
+            const int off = row * kRowB + ch * 16;
+            *reinterpret_cast<u32x4*>(base + off) = stage_x[s];
+            *reinterpret_cast<u32x4*>(base + kTileB + off) = stage_h[s];
+            *reinterpret_cast<u32x4*>(base + 2 * kTileB + off) = stage_l[s];
+        }
+    };
+
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+    const int ksteps = static_cast<int>(np / kBK);
+    load_global(0);
+    store_lds(0);
+    __syncthreads();
+    // Fragment rows within the tiles: the wave's 128 rows of L (m) and of R (c).
+    const int frag_m = wm * 128 + (lane & 31);
+    const int frag_c = wn * 128 + (lane & 31);
+    const int frag_k = (lane >> 5) * 16;  // byte offset of the lane's 8 k values
+    for (int kt = 0; kt < ksteps; ++kt) {
+        const bool more = kt + 1 < ksteps;
+        if (more) load_global(static_cast<int64_t>(kt + 1) * kBK);
+        const char* base = lds + (kt & 1) * kStageB;
+        const char* ltile = SPLIT_L ? base + kTileB : base;  // L: hi (split) or single
+        const char* ltile_lo = base + 2 * kTileB;
+        const char* rtile = SPLIT_L ? base : base + kTileB;  // R: single or hi
+        const char* rtile_lo = base + 2 * kTileB;
+#pragma unroll
+        for (int ks = 0; ks < kBK / 16; ++ks) {
+            const int kb = ks * 32 + frag_k;
+            bf16x8 a[4], b[4], s_lo[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                a[t] = *reinterpret_cast<const bf16x8*>(ltile + (frag_m + t * 32) * kRowB + kb);
+                b[t] = *reinterpret_cast<const bf16x8*>(rtile + (frag_c + t * 32) * kRowB + kb);
+                if (SPLIT_L)
+                    s_lo[t] = *reinterpret_cast<const bf16x8*>(ltile_lo + (frag_m + t * 32) * kRowB + kb);
+                else
+                    s_lo[t] = *reinterpret_cast<const bf16x8*>(rtile_lo + (frag_c + t * 32) * kRowB + kb);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    if (SPLIT_L)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s_lo[i], b[j], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], s_lo[j], acc[i][j], 0, 0, 0);
+                }
+        }
+        if (more) store_lds((kt + 1) & 1);
+        __syncthreads();
+    }
+
+    // Epilogue.  Accumulator r of tile (i, j) is element
+    // (m0 + wm*128 + 32 i + 8 (r/4) + 4 (lane/32) + r%4, c0 + wn*128 + 32 j + lane%32).
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t m = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            float4 fsm = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (EPI == 1 && ep.fs && m < n) fsm = *reinterpret_cast<const float4*>(ep.fs + 4 * m);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t c = c0 + wn * 128 + j * 32 + (lane & 31);
+                const float y = acc[i][j][r] * scale;
+                if (EPI == 0) {
+                    uint16_t h, l;
+                    split_bf16(y, h, l);
+                    ep.y_hi[m * np + c] = h;
+                    ep.y_lo[m * np + c] = l;
+                } else if (m < n && c < n) {
+                    float g = y;
+                    if (ep.fs) {
+                        const float4 b = *reinterpret_cast<const float4*>(ep.ft + 4 * c);
+                        const float d0 = fsm.x - b.x, d1 = fsm.y - b.y, d2 = fsm.z - b.z, d3 = fsm.w - b.w;
+                        g += sqrtf(fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3))));
+                    }
+                    if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[m * n + c];
+                    ep.G[m * n + c] = g;
+                }
+            }
+        }
+    }
+}
+
+template <bool SPLIT_L, int EPI>
+int launch_gemm(const uint16_t* X, const uint16_t* Sh, const uint16_t* Sl, int n, int64_t np, float scale,
+                const GradEpi& ep, hipStream_t s, const char* name) {
+    static bool attr = false;
+    if (!attr) {
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_nt<SPLIT_L, EPI>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, kLdsB));
+        attr = true;
+    }
+    const int tiles_1d = static_cast<int>(np / kTile);
+    const int64_t tiles = static_cast<int64_t>(tiles_1d) * tiles_1d;
+    const unsigned blocks = static_cast<unsigned>(div_up(tiles, kNumXcd) * kNumXcd);
+    hipLaunchKernelGGL((gemm_nt<SPLIT_L, EPI>), dim3(blocks), dim3(kThreads), kLdsB, s, X, Sh, Sl, n, np, tiles_1d,
+                       scale, ep);
+    return check_launch(name);
+}
+
+int64_t padded(int64_t n) { return round_up(n, kTile); }
+
+}  // namespace
+}  // namespace cg
+
+using namespace cg;
+
+extern "C" {
+
+size_t cg_dense_adjacency_elements(int64_t n) {
+    if (n <= 0) return 0;
+    const int64_t np = padded(n);
+    return static_cast<size_t>(np) * static_cast<size_t>(np);
+}
+
+int cg_adjacency_densify_bf16(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint16_t* out,
+                              cg_stream_t stream) {
+    if (!col_indices || !row_pointers || !out || n <= 0 || n > (int64_t(1) << 20))
+        return fail(CG_EINVAL, "cg_adjacency_densify_bf16: bad arguments");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t np = padded(n);
+    CG_CHECK_HIP(hipMemsetAsync(out, 0, sizeof(uint16_t) * static_cast<size_t>(np * np), s));
+    hipLaunchKernelGGL(densify_rows, dim3(static_cast<unsigned>(n)), dim3(256), 0, s, col_indices, row_pointers,
+                       static_cast<int>(n), np, out);
+    return check_launch("densify_rows");
+}
+
+size_t cg_gradient_mfma_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    const size_t e = cg_dense_adjacency_elements(n);
+    Carver cv{nullptr, 0, ~size_t(0)};
+    cv.take<uint16_t>(e);
+    cv.take<uint16_t>(e);
+    cv.take<uint16_t>(e);
+    cv.take<uint16_t>(e);
+    return cv.used + 256;
+}
+
+int cg_gradient_mfma(const uint16_t* a_dense, const uint16_t* b_dense, int64_t n, const float* P,
+                     const float* source_features, const float* target_features, int lambda_,
+                     float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                     cg_stream_t stream) {
+    if (!a_dense || !b_dense || !P || !gradient || n <= 0 || n > (int64_t(1) << 20))
+        return fail(CG_EINVAL, "cg_gradient_mfma: bad arguments");
+    if ((source_features == nullptr) != (target_features == nullptr))
+        return fail(CG_EINVAL, "cg_gradient_mfma: give both feature matrices or neither");
+    if (source_features && (!aligned16(source_features) || !aligned16(target_features)))
+        return fail(CG_EINVAL, "cg_gradient_mfma: feature matrices must be 16-byte aligned");
+    if (!aligned16(a_dense) || !aligned16(b_dense))
+        return fail(CG_EINVAL, "cg_gradient_mfma: dense adjacencies must be 16-byte aligned");
+    if (!workspace || workspace_bytes < cg_gradient_mfma_workspace(n))
+        return fail(CG_EWORKSPACE, "cg_gradient_mfma: workspace too small");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int ni = static_cast<int>(n);
+    const int64_t np = padded(n);
+    const size_t e = static_cast<size_t>(np * np);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    uint16_t* pt_hi = cv.take<uint16_t>(e);
+    uint16_t* pt_lo = cv.take<uint16_t>(e);
+    uint16_t* y_hi = cv.take<uint16_t>(e);
+    uint16_t* y_lo = cv.take<uint16_t>(e);
+    const unsigned tb = static_cast<unsigned>(np / 64);
+    hipLaunchKernelGGL(split_transpose, dim3(tb, tb), dim3(256), 0, s, P, ni, np, pt_hi, pt_lo);
+    if (int rc = check_launch("split_transpose")) return rc;
+    GradEpi ep1{nullptr, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo};
+    if (int rc = launch_gemm<false, 0>(a_dense, pt_hi, pt_lo, ni, np, 1.0f, ep1, s, "gemm_nt<Y = A P>")) return rc;
+    const float rl = regularizer_factor * static_cast<float>(lambda_);
+    GradEpi ep2{rl != 0.0f ? P : nullptr, source_features, target_features, rl, gradient, nullptr, nullptr};
+    return launch_gemm<true, 1>(b_dense, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
+}
+
+}  // extern "C"

@@ -39,27 +39,40 @@ def _check_config(config: Config) -> None:
         raise NotImplementedError("the HIP kernels compute in FP32: set Config(dtype=torch.float32)")
     if config.sinkhorn_method != SinkhornMethod.LOG:
         raise NotImplementedError("the HIP Sinkhorn is the log-domain one: set sinkhorn_method=SinkhornMethod.LOG")
-    if config.gradient_precision != "fp32":
+    if config.gradient_precision not in ("fp32", "bf16x2"):
         raise NotImplementedError(f"gradient_precision {config.gradient_precision!r} is not available")
 
 
 def _gradient_into(
-    out: torch.Tensor, A: Adjacency, B: Adjacency, P: torch.Tensor, features, iteration: int, factor: float
+    out: torch.Tensor, A: Adjacency, B: Adjacency, P: torch.Tensor, features, iteration: int, factor: float,
+    precision: str = "fp32",
 ) -> torch.Tensor:
     """out = -2 A P B + D + factor * iteration * (1 - 2P) for symmetric A, B.
     `features` is a Features (distance recomputed in the kernel), an n x n
-    distance tensor (added afterwards) or None."""
+    distance tensor (added afterwards) or None.  precision "fp32": CSR
+    adjacency times FP32 P (cg_gradient); "bf16x2": dense 0/1 BF16
+    adjacency times P split into BF16 hi + lo on the matrix cores, FP32
+    accumulation (cg_gradient_mfma)."""
     n = P.shape[0]
     dev = P.device
     fs = ft = None
     if isinstance(features, Features):
         fs, ft = features.source.contiguous(), features.target.contiguous()
-    ws = nat.workspace(dev, nat.query("cg_gradient_workspace", n), "gradient")
-    nat.call(
-        "cg_gradient", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(B.col_indices),
-        nat.ptr(B.row_pointers), n, nat.ptr(P), nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
-        nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
-    )
+    if precision == "bf16x2":
+        ws = nat.workspace(dev, nat.query("cg_gradient_mfma_workspace", n), "gradient_mfma")
+        nat.call(
+            "cg_gradient_mfma", nat.ptr(A.dense_bf16()), nat.ptr(B.dense_bf16()), n, nat.ptr(P), nat.ptr(fs),
+            nat.ptr(ft), int(iteration), float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+        )
+    elif precision == "fp32":
+        ws = nat.workspace(dev, nat.query("cg_gradient_workspace", n), "gradient")
+        nat.call(
+            "cg_gradient", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(B.col_indices),
+            nat.ptr(B.row_pointers), n, nat.ptr(P), nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
+            nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+        )
+    else:
+        raise NotImplementedError(f"gradient precision {precision!r}")
     if isinstance(features, torch.Tensor):
         out += features
     return out
@@ -156,7 +169,7 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
     for lam in range(config.iter_count):
         for it in range(1, config.frank_wolfe_iter_count + 1):
             start_time = TimeStamp(config.device)
-            _gradient_into(gradient, A, B, P, features, lam, factor)
+            _gradient_into(gradient, A, B, P, features, lam, factor, config.gradient_precision)
             profile.log_time(start_time, Phase.GRADIENT)
 
             start_time = TimeStamp(config.device)

@@ -199,6 +199,25 @@ int cg_sinkhorn_unrolled_backward(const float* C, int64_t n, float regularizatio
                                   const float* weights, float lam, double* grad_lam_w, void* workspace,
                                   size_t workspace_bytes, cg_stream_t stream);
 
+/* The precision form of cg_gradient (same quantity, cugal/pred.py:38-74):
+ * G = -2 A P B + D + regularizer_factor * lambda_ * (1 - 2P) with A, B as
+ * dense 0/1 BF16 matrices (exact) and P split into BF16 hi + lo, products
+ * on the matrix cores (v_mfma_f32_32x32x16_bf16) accumulated in FP32.
+ * Where the reference multiplies dense adjacencies in FP32 (torch.matmul,
+ * cugal/pred.py:45-46) this reads half the bytes per operand and runs on
+ * MFMA; each product carries a relative error of about 2^-16 from the hi+lo
+ * split.  a_dense, b_dense: cg_dense_adjacency_elements(n) uint16 (BF16
+ * bits) each, from cg_adjacency_densify_bf16: the adjacency zero padded to
+ * np = n rounded up to 256 (np x np, row-major).  Undirected graphs only. */
+size_t cg_dense_adjacency_elements(int64_t n);
+int cg_adjacency_densify_bf16(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint16_t* out,
+                              cg_stream_t stream);
+size_t cg_gradient_mfma_workspace(int64_t n);
+int cg_gradient_mfma(const uint16_t* a_dense, const uint16_t* b_dense, int64_t n, const float* P,
+                     const float* source_features, const float* target_features, int lambda_,
+                     float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                     cg_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Frank-Wolfe update.
  * ---------------------------------------------------------------------- */
