@@ -97,8 +97,10 @@ struct GradEpi {
 };
 
 // The grid has kNumXcd * ceil(tiles / kNumXcd) blocks; returns false for
-// the surplus ones.
-__device__ __forceinline__ bool tile_of_block(int64_t bid, int tiles_1d, int& tm, int& tn) {
+// the surplus ones.  Groups run along the first index (ta), which the
+// caller binds to the split operand's tiles: 4 of its hi + lo panel pairs
+// and 8 panels of the single operand per XCD.
+__device__ __forceinline__ bool tile_of_block(int64_t bid, int tiles_1d, int& ta, int& tb) {
     const int64_t tiles = static_cast<int64_t>(tiles_1d) * tiles_1d;
     const int64_t per = div_up(tiles, kNumXcd);
     const int64_t t = (bid % kNumXcd) * per + bid / kNumXcd;  // an XCD's blocks: consecutive t
@@ -108,8 +110,8 @@ __device__ __forceinline__ bool tile_of_block(int64_t bid, int tiles_1d, int& tm
     const int64_t first = g * kGroup;
     const int rows = static_cast<int>(min<int64_t>(kGroup, tiles_1d - first));
     const int64_t w = t % in_group;
-    tm = static_cast<int>(first + w % rows);
-    tn = static_cast<int>(w / rows);
+    ta = static_cast<int>(first + w % rows);
+    tb = static_cast<int>(w / rows);
     return true;
 }
 
@@ -126,8 +128,10 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    int tm, tn;
-    if (!tile_of_block(blockIdx.x, tiles_1d, tm, tn)) return;
+    int ta, tb;
+    if (!tile_of_block(blockIdx.x, tiles_1d, ta, tb)) return;
+    const int tm = SPLIT_L ? ta : tb;
+    const int tn = SPLIT_L ? tb : ta;
     const int64_t m0 = static_cast<int64_t>(tm) * kTile;
     const int64_t c0 = static_cast<int64_t>(tn) * kTile;
     // Rows of the three tiles: the single operand's rows and the split
@@ -171,47 +175,88 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
     const int ksteps = static_cast<int>(np / kBK);
-    load_global(0);
-    store_lds(0);
-    __syncthreads();
     // Fragment rows within the tiles: the wave's 128 rows of L (m) and of R (c).
     const int frag_m = wm * 128 + (lane & 31);
     const int frag_c = wn * 128 + (lane & 31);
     const int frag_k = (lane >> 5) * 16;  // byte offset of the lane's 8 k values
-    for (int kt = 0; kt < ksteps; ++kt) {
-        const bool more = kt + 1 < ksteps;
-        if (more) load_global(static_cast<int64_t>(kt + 1) * kBK);
-        const char* base = lds + (kt & 1) * kStageB;
-        const char* ltile = SPLIT_L ? base + kTileB : base;  // L: hi (split) or single
-        const char* ltile_lo = base + 2 * kTileB;
-        const char* rtile = SPLIT_L ? base : base + kTileB;  // R: single or hi
-        const char* rtile_lo = base + 2 * kTileB;
+    // The single-operand fragments (4), the split operand's hi (4) and lo (4)
+    // of one 16-wide k step.
+    auto read_frags = [&](const char* base, int ks, bf16x8(&x)[4], bf16x8(&h)[4], bf16x8(&l)[4]) {
+        const int kb = ks * 32 + frag_k;
+        const int xr = SPLIT_L ? frag_c : frag_m;
+        const int sr = SPLIT_L ? frag_m : frag_c;
 #pragma unroll
-        for (int ks = 0; ks < kBK / 16; ++ks) {
-            const int kb = ks * 32 + frag_k;
-            bf16x8 a[4], b[4], s_lo[4];
+        for (int t = 0; t < 4; ++t) {
+            x[t] = *reinterpret_cast<const bf16x8*>(base + (xr + t * 32) * kRowB + kb);
+            h[t] = *reinterpret_cast<const bf16x8*>(base + kTileB + (sr + t * 32) * kRowB + kb);
+            l[t] = *reinterpret_cast<const bf16x8*>(base + 2 * kTileB + (sr + t * 32) * kRowB + kb);
+        }
+    };
+    // All hi products, then all lo products: a tile's two MFMAs are 16
+    // instructions apart, beyond the accumulator latency.
+    // MFMAs [first, first + count) of the step's 32.
+    auto mfma_step = [&](const bf16x8(&x)[4], const bf16x8(&h)[4], const bf16x8(&l)[4], int first, int count) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                a[t] = *reinterpret_cast<const bf16x8*>(ltile + (frag_m + t * 32) * kRowB + kb);
-                b[t] = *reinterpret_cast<const bf16x8*>(rtile + (frag_c + t * 32) * kRowB + kb);
-                if (SPLIT_L)
-                    s_lo[t] = *reinterpret_cast<const bf16x8*>(ltile_lo + (frag_m + t * 32) * kRowB + kb);
-                else
-                    s_lo[t] = *reinterpret_cast<const bf16x8*>(rtile_lo + (frag_c + t * 32) * kRowB + kb);
-            }
+        for (int half = 0; half < 2; ++half)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    const int idx = half * 16 + i * 4 + j;
+                    if (idx < first || idx >= first + count) continue;
+                    const bf16x8& s = half ? (SPLIT_L ? l[i] : l[j]) : (SPLIT_L ? h[i] : h[j]);
                     if (SPLIT_L)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s_lo[i], b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s, x[j], acc[i][j], 0, 0, 0);
                     else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], s_lo[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x[i], s, acc[i][j], 0, 0, 0);
                 }
+    };
+
+    // Pipeline: stage kt & 1 holds k slice kt; the staging registers hold
+    // slice kt + 1, which is written to the other stage during the second
+    // k step while slice kt + 2 is fetched.  The last iterations re-fetch
+    // the final slice (a harmless, in-bounds read) so that the loop body is
+    // one basic block the scheduling groups below can order.
+    load_global(0);
+    store_lds(0);
+    load_global(static_cast<int64_t>(min(1, ksteps - 1)) * kBK);
+    __syncthreads();
+    bf16x8 fx[4], fh[4], fl[4], gx[4], gh[4], gl[4];
+    read_frags(lds, 0, fx, fh, fl);
+    for (int kt = 0; kt < ksteps; ++kt) {
+        const char* base = lds + (kt & 1) * kStageB;
+        read_frags(base, 1, gx, gh, gl);
+        mfma_step(fx, fh, fl, 0, 32);
+        store_lds((kt + 1) & 1);
+        load_global(static_cast<int64_t>(min(kt + 2, ksteps - 1)) * kBK);
+        // The first 24 MFMAs of the second k step, then the barrier (the
+        // next stage is complete), then the next slice's first fragments are
+        // read under the last 8 MFMAs.  After the last slice those reads
+        // hit a stage that is no longer used (in bounds, discarded).
+        mfma_step(gx, gh, gl, 0, 24);
+        // Order: the second step's 12 LDS reads spread over the first step's
+        // MFMAs; the 12 LDS writes of slice kt + 1 and the 12 global loads
+        // of slice kt + 2 over the second step's.
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
-        if (more) store_lds((kt + 1) & 1);
+        __builtin_amdgcn_sched_group_barrier(0x008, 20, 0);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
         __syncthreads();
+        read_frags(lds + ((kt + 1) & 1) * kStageB, 0, fx, fh, fl);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma_step(gx, gh, gl, 24, 8);
     }
 
     // Epilogue.  Accumulator r of tile (i, j) is element
