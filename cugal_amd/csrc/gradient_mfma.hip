@@ -19,8 +19,14 @@
 // fragment is one 16-byte read.  A block computes a 256 x 256 tile with four
 // waves of 128 x 128 (v_mfma_f32_32x32x16_bf16, 4 x 4 tiles of 32 x 32),
 // staging 32-wide k slices of the three input tiles (single operand, hi, lo)
-// in two LDS buffers with an 80-byte row stride (conflict-free 16-byte
-// reads).  Tiles are ordered in groups of 4 tile rows and dealt so that the
+// in three LDS buffers filled by direct global -> LDS loads
+// (global_load_lds_dwordx4: lane i of a wave lands at byte 16 i of the
+// wave's 1 KiB destination, so each lane fetches the chunk whose swizzled
+// slot that is).  A tile row is 64 bytes (four 16-byte chunks); chunk
+// c of row r sits at chunk position c ^ f(r), f(r) = ((r >> 1) + (r >> 3))
+// & 3, which makes both the stores (4 lanes per row) and the fragment reads
+// (consecutive rows, one chunk) hit distinct banks in every 8- or 16-lane
+// group.  Tiles are ordered in groups of 4 tile rows and dealt so that the
 // 32 blocks one XCD runs together share 4 row panels and 8 column panels
 // in its L2.
 #include "common.hpp"
@@ -36,12 +42,16 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 constexpr int kTile = 256;      // block tile (rows and columns)
 constexpr int kBK = 32;         // k slice per LDS stage
 constexpr int kThreads = 256;   // four waves
-constexpr int kRowB = 80;       // LDS bytes per tile row: 64 + 16 padding
+constexpr int kRowB = 64;       // LDS bytes per tile row (32 BF16), chunks swizzled
 constexpr int kTileB = kTile * kRowB;           // one operand tile in LDS
 constexpr int kStageB = 3 * kTileB;             // single + hi + lo
-constexpr int kLdsB = 2 * kStageB;              // two stages: 120 KiB
+constexpr int kStages = 3;
+constexpr int kLdsB = kStages * kStageB;        // three stages: 144 KiB
 constexpr int kGroup = 4;       // tile rows per group in the block order
 constexpr int kNumXcd = 8;
+
+// LDS chunk swizzle of a tile row (see the file comment).
+__device__ __forceinline__ int swz(int row) { return ((row >> 1) + (row >> 3)) & 3; }
 
 __device__ __forceinline__ uint16_t bf16_bits(__bf16 x) { return __builtin_bit_cast(uint16_t, x); }
 
@@ -139,30 +149,28 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__
     const int64_t x_row0 = SPLIT_L ? c0 : m0;
     const int64_t s_row0 = SPLIT_L ? m0 : c0;
 
-    // Global -> LDS: per stage, each tile is 256 rows x 4 chunks of 16 B;
-    // thread t moves chunks t, t + 256, t + 512, t + 768 of each.
-    u32x4 stage_x[4], stage_h[4], stage_l[4];
-    auto load_global = [&](int64_t k0) {
+    // Global -> LDS: per stage, each tile is 256 rows x 64 bytes = 16 KiB,
+    // four 1 KiB wave loads per thread block instruction.  Block
+    // instruction s of wave w fills bytes [1024 (w + 4 s), +1024) of a tile:
+    // lane l lands in row r = 16 (w + 4 s) + l / 4 at chunk slot l % 4,
+    // which holds logical chunk (l % 4) ^ swz(r); the four lanes of a row
+    // read its whole 64-byte segment.
+    const int lrow = wave * 16 + (lane >> 2);
+    const int lchunk = (lane & 3) ^ swz(lrow);  // swz(r + 64 s) == swz(r)
+    auto issue_loads = [&](int64_t k0, int stage) {
+        char* base = lds + stage * kStageB + wave * 1024;
+        const int64_t kx = k0 + lchunk * 8;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int q = tid + s * kThreads;
-            const int row = q >> 2, ch = q & 3;
-            const int64_t kx = k0 + ch * 8;
-            stage_x[s] = *reinterpret_cast<const u32x4*>(X + (x_row0 + row) * np + kx);
-            stage_h[s] = *reinterpret_cast<const u32x4*>(Sh + (s_row0 + row) * np + kx);
-            stage_l[s] = *reinterpret_cast<const u32x4*>(Sl + (s_row0 + row) * np + kx);
-        }
-    };
-    auto store_lds = [&](int stage) {
-        char* base = lds + stage * kStageB;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int q = tid + s * kThreads;
-            const int row = q >> 2, ch = q & 3;
-            const int off = row * kRowB + ch * 16;
-            *reinterpret_cast<u32x4*>(base + off) = stage_x[s];
-            *reinterpret_cast<u32x4*>(base + kTileB + off) = stage_h[s];
-            *reinterpret_cast<u32x4*>(base + 2 * kTileB + off) = stage_l[s];
+            const int64_t r = lrow + 64 * s;
+            __builtin_amdgcn_global_load_lds(X + (x_row0 + r) * np + kx,
+                                             (__attribute__((address_space(3))) void*)(base + s * 4096), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(Sh + (s_row0 + r) * np + kx,
+                                             (__attribute__((address_space(3))) void*)(base + kTileB + s * 4096), 16,
+                                             0, 0);
+            __builtin_amdgcn_global_load_lds(Sl + (s_row0 + r) * np + kx,
+                                             (__attribute__((address_space(3))) void*)(base + 2 * kTileB + s * 4096),
+                                             16, 0, 0);
         }
     };
 
@@ -178,18 +186,24 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__
     // Fragment rows within the tiles: the wave's 128 rows of L (m) and of R (c).
     const int frag_m = wm * 128 + (lane & 31);
     const int frag_c = wn * 128 + (lane & 31);
-    const int frag_k = (lane >> 5) * 16;  // byte offset of the lane's 8 k values
+    // The lane's 8 k values are chunk 2 ks + lane / 32 of its row; the
+    // swizzle is the same for rows 32 apart, so one offset per k step.
+    const int frag_sub = lane >> 5;
+    const int swz_x = swz(SPLIT_L ? frag_c : frag_m);
+    const int swz_s = swz(SPLIT_L ? frag_m : frag_c);
     // The single-operand fragments (4), the split operand's hi (4) and lo (4)
     // of one 16-wide k step.
     auto read_frags = [&](const char* base, int ks, bf16x8(&x)[4], bf16x8(&h)[4], bf16x8(&l)[4]) {
-        const int kb = ks * 32 + frag_k;
+        const int ch = 2 * ks + frag_sub;
+        const int kx = (ch ^ swz_x) << 4;
+        const int ksw = (ch ^ swz_s) << 4;
         const int xr = SPLIT_L ? frag_c : frag_m;
         const int sr = SPLIT_L ? frag_m : frag_c;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            x[t] = *reinterpret_cast<const bf16x8*>(base + (xr + t * 32) * kRowB + kb);
-            h[t] = *reinterpret_cast<const bf16x8*>(base + kTileB + (sr + t * 32) * kRowB + kb);
-            l[t] = *reinterpret_cast<const bf16x8*>(base + 2 * kTileB + (sr + t * 32) * kRowB + kb);
+            x[t] = *reinterpret_cast<const bf16x8*>(base + (xr + t * 32) * kRowB + kx);
+            h[t] = *reinterpret_cast<const bf16x8*>(base + kTileB + (sr + t * 32) * kRowB + ksw);
+            l[t] = *reinterpret_cast<const bf16x8*>(base + 2 * kTileB + (sr + t * 32) * kRowB + ksw);
         }
     };
     // All hi products, then all lo products: a tile's two MFMAs are 16
@@ -217,47 +231,46 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint16_t* __restrict__
     // k step while slice kt + 2 is fetched.  The last iterations re-fetch
     // the final slice (a harmless, in-bounds read) so that the loop body is
     // one basic block the scheduling groups below can order.
-    load_global(0);
-    store_lds(0);
-    load_global(static_cast<int64_t>(min(1, ksteps - 1)) * kBK);
+    // Pipeline: slice kt lives in stage kt % 3.  Iteration kt starts the
+    // loads of slice kt + 2, computes slice kt, then waits for slice kt + 1
+    // (all but the 12 newest loads) and synchronises.  The last iterations
+    // re-fetch the final slice into a stage no longer read (in bounds), so
+    // the load count per iteration, and the wait, stay fixed.
+    issue_loads(0, 0);
+    issue_loads(static_cast<int64_t>(min(1, ksteps - 1)) * kBK, 1);
+    asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
     __syncthreads();
     bf16x8 fx[4], fh[4], fl[4], gx[4], gh[4], gl[4];
     read_frags(lds, 0, fx, fh, fl);
+    int stage = 0;
     for (int kt = 0; kt < ksteps; ++kt) {
-        const char* base = lds + (kt & 1) * kStageB;
+        const int stage1 = stage == kStages - 1 ? 0 : stage + 1;
+        const int stage2 = stage1 == kStages - 1 ? 0 : stage1 + 1;
+        issue_loads(static_cast<int64_t>(min(kt + 2, ksteps - 1)) * kBK, stage2);
+        const char* base = lds + stage * kStageB;
         read_frags(base, 1, gx, gh, gl);
         mfma_step(fx, fh, fl, 0, 32);
-        store_lds((kt + 1) & 1);
-        load_global(static_cast<int64_t>(min(kt + 2, ksteps - 1)) * kBK);
-        // The first 24 MFMAs of the second k step, then the barrier (the
-        // next stage is complete), then the next slice's first fragments are
-        // read under the last 8 MFMAs.  After the last slice those reads
-        // hit a stage that is no longer used (in bounds, discarded).
         mfma_step(gx, gh, gl, 0, 24);
-        // Order: the second step's 12 LDS reads spread over the first step's
-        // MFMAs; the 12 LDS writes of slice kt + 1 and the 12 global loads
-        // of slice kt + 2 over the second step's.
+        // Order: the global loads first, then the second step's 12 LDS
+        // reads spread over the first step's MFMAs.
+        __builtin_amdgcn_sched_group_barrier(0x020, 12, 0);
 #pragma unroll
         for (int q = 0; q < 12; ++q) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x008, 20, 0);
-#pragma unroll
-        for (int q = 0; q < 12; ++q) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < 12; ++q) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-        }
-        __syncthreads();
-        read_frags(lds + ((kt + 1) & 1) * kStageB, 0, fx, fh, fl);
+        __builtin_amdgcn_sched_group_barrier(0x008, 44, 0);
+        // Slice kt + 1 has landed and this wave's reads of the stage the
+        // next iteration refills are done; no fence, so the 12 newest loads
+        // stay in flight across the barrier.
+        asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        read_frags(lds + stage1 * kStageB, 0, fx, fh, fl);
         __builtin_amdgcn_sched_barrier(0);
         mfma_step(gx, gh, gl, 24, 8);
+        stage = stage1;
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
     // Epilogue.  Accumulator r of tile (i, j) is element
     // (m0 + wm*128 + 32 i + 8 (r/4) + 4 (lane/32) + r%4, c0 + wn*128 + 32 j + lane%32).
