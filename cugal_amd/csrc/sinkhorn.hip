@@ -326,6 +326,55 @@ __device__ float lehmann_momentum(const double* errors, int nerr, int eval_freq,
 constexpr int kCombCols = 64;
 constexpr int kCombWaves = 4;
 
+// The error bookkeeping shared by the combines: wave 0 of each block holds
+// the per-column error terms; the blocks' FP64 sums are totalled in fixed
+// order by the last block to arrive, which records the error, raises `done`
+// below the threshold and otherwise updates the Lehmann factor.
+__device__ void combine_error(double err, int want_err, int check_index, int cur_buf, double threshold,
+                              int eval_freq, int momentum_start, double* __restrict__ errpart,
+                              SinkCtl* __restrict__ ctl) {
+    __shared__ double dred[kCombWaves];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & (kCombCols - 1);
+    const int wave = threadIdx.x / kCombCols;
+    if (!want_err) return;
+    if (wave == 0) {
+        err = wave_sum(err);
+        if (lane == 0) dred[0] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        errpart[blockIdx.x] = dred[0];
+        __threadfence();
+        const unsigned t = atomicAdd(&ctl->ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double total = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x) total += errpart[b];
+    // Fixed-shape reduction over the block: deterministic.
+    total = wave_sum(total);
+    if ((threadIdx.x & 63) == 0) dred[threadIdx.x / 64] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kCombWaves; ++w) sum += dred[w];
+        double* errors = ctl_errors(ctl);
+        errors[ctl->nerr] = sum;
+        ctl->nerr += 1;
+        ctl->ticket = 0u;
+        if (sum < threshold) {
+            ctl->done = 1;
+            ctl->iterations = check_index + 1;
+            ctl->final_buf = cur_buf;
+        } else {
+            ctl->momentum = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
+        }
+    }
+}
+
 template <bool PAIRS>
 __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
     const void* __restrict__ part_raw, int nblk, int n, const float* __restrict__ v2_in, float* __restrict__ v2_out,
@@ -333,8 +382,6 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
     __shared__ float rm[kCombWaves][kCombCols];
     __shared__ double rs[kCombWaves][kCombCols];
-    __shared__ double dred[kCombWaves];
-    __shared__ int is_last;
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
@@ -379,43 +426,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         v2_out[j] = static_cast<float>(raw_out ? (L - vj) : (vj - L));
         if (want_err) err = fabs(exp2(L) - 1.0);
     }
-    if (!want_err) return;
-    // Block sum of the error (wave 0 holds all terms).
-    if (wave == 0) {
-        err = wave_sum(err);
-        if (lane == 0) dred[0] = err;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        errpart[blockIdx.x] = dred[0];
-        __threadfence();
-        const unsigned t = atomicAdd(&ctl->ticket, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double total = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x) total += errpart[b];
-    // Fixed-shape reduction over the block: deterministic.
-    total = wave_sum(total);
-    if ((threadIdx.x & 63) == 0) dred[threadIdx.x / 64] = total;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < kCombWaves; ++w) sum += dred[w];
-        double* errors = ctl_errors(ctl);
-        errors[ctl->nerr] = sum;
-        ctl->nerr += 1;
-        ctl->ticket = 0u;
-        if (sum < threshold) {
-            ctl->done = 1;
-            ctl->iterations = check_index + 1;
-            ctl->final_buf = cur_buf;
-        } else {
-            ctl->momentum = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
-        }
-    }
+    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl);
 }
 
 // v_next = (1 - m) v - m * lse  (only when Lehmann momentum is configured).
@@ -427,6 +438,230 @@ __global__ void sk_apply_momentum(const float* __restrict__ v2_in, float* __rest
     const float m = ctl->momentum;
     const float lse = v2_io[j];
     v2_io[j] = (m == 1.0f) ? -lse : (1.0f - m) * v2_in[j] - m * lse;
+}
+
+// ---------------------------------------------------------------------------
+// STANDARD Sinkhorn, cugal/sinkhorn.py:122-144: with K = exp(-C/reg),
+//   v_j = v_j^(1-m) (1 / (sum_i u_i K_ij + 1e-16))^m,
+//   u_i = u_i^(1-m) (1 / (sum_j K_ij v_j + 1e-16))^m,
+// and the column-marginal error sum_j |sum_i u_i K_ij v_j - 1| at
+// iterations 0, f, 2f, ...  K is recomputed as exp2(C * scale2) where it is
+// read (never stored).  The fused sweep is the log one with plain sums: per
+// row, s_i = sum_j K_ij v_j gives u_i, and with the row's K still in
+// registers the column sums sum_i u_i K_ij of the new u accumulate.
+// ---------------------------------------------------------------------------
+constexpr float kStdEps = 1e-16f;  // M_EPS, cugal/sinkhorn.py:12
+
+__device__ __forceinline__ float std_blend(float old, float fresh, float mom) {
+    return mom == 1.0f ? fresh : powf(old, 1.0f - mom) * powf(fresh, mom);
+}
+
+template <int T, int G>
+__device__ __forceinline__ void std_fused_row(float4 (&r)[G], float (&cs)[4 * G], int i, int n, float scale2,
+                                              const float* __restrict__ sv, float* __restrict__ red,
+                                              float* __restrict__ u, float mom) {
+    constexpr int W = T / kWave;
+    const int tid = threadIdx.x;
+    const float u_old = u[i];
+    float ls = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (col < n) {
+            const float4 vv = *reinterpret_cast<const float4*>(sv + col);
+            r[g].x = exp2_fast(r[g].x * scale2);
+            r[g].y = exp2_fast(r[g].y * scale2);
+            r[g].z = exp2_fast(r[g].z * scale2);
+            r[g].w = exp2_fast(r[g].w * scale2);
+            ls += (r[g].x * vv.x + r[g].y * vv.y) + (r[g].z * vv.z + r[g].w * vv.w);
+        }
+    }
+    ls = wave_sum(ls);
+    float* slot = red + (i & 1) * W;
+    if ((tid & (kWave - 1)) == 0) slot[tid / kWave] = ls;
+    __syncthreads();
+    float S = 0.0f;
+#pragma unroll
+    for (int k = 0; k < W; ++k) S += slot[k];
+    const float ui = std_blend(u_old, 1.0f / (S + kStdEps), mom);
+    if (tid == 0) u[i] = ui;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if ((g * T + tid) * 4 < n) {
+            cs[4 * g + 0] = fmaf(ui, r[g].x, cs[4 * g + 0]);
+            cs[4 * g + 1] = fmaf(ui, r[g].y, cs[4 * g + 1]);
+            cs[4 * g + 2] = fmaf(ui, r[g].z, cs[4 * g + 2]);
+            cs[4 * g + 3] = fmaf(ui, r[g].w, cs[4 * g + 3]);
+        }
+    }
+}
+
+template <int T, int G>
+__global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C, int n, float scale2,
+                                                    const float* __restrict__ v, float* __restrict__ u,
+                                                    float* __restrict__ part, int rows_per_blk,
+                                                    const SinkCtl* __restrict__ ctl) {
+    extern __shared__ float smem[];
+    if (ctl->done) return;
+    const float mom = ctl->momentum;
+    const int tid = threadIdx.x;
+    float* sv = smem;
+    float* red = smem + round_up(n, 4);
+    for (int j = tid * 4; j < n; j += T * 4)
+        *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v + j);
+    __syncthreads();
+    float cs[4 * G];
+#pragma unroll
+    for (int k = 0; k < 4 * G; ++k) cs[k] = 0.0f;
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    const int64_t ld = n;
+    if constexpr (4 * G * T <= 24 * 1024) {
+        float4 a[G], b[G];
+        if (r0 < r1) fused_load_row<T, G>(a, C + r0 * ld, n);
+        for (int i = r0; i < r1; i += 2) {
+            if (i + 1 < r1) fused_load_row<T, G>(b, C + (i + 1) * ld, n);
+            std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
+            if (i + 1 >= r1) break;
+            if (i + 2 < r1) fused_load_row<T, G>(a, C + (i + 2) * ld, n);
+            std_fused_row<T, G>(b, cs, i + 1, n, scale2, sv, red, u, mom);
+        }
+    } else {
+        float4 a[G];
+        for (int i = r0; i < r1; ++i) {
+            fused_load_row<T, G>(a, C + i * ld, n);
+            std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
+        }
+    }
+    float* out = part + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (col < n)
+            *reinterpret_cast<float4*>(out + col) = make_float4(cs[4 * g], cs[4 * g + 1], cs[4 * g + 2], cs[4 * g + 3]);
+    }
+}
+
+// Column sweep: block (bx, by) sums u_i K_ij over rows [bx*rpb, ...) for
+// its column chunk.
+template <int T, int CPT>
+__global__ __launch_bounds__(T) void std_col_sweep(const float* __restrict__ C, int n, int64_t ldc, float scale2,
+                                                  const float* __restrict__ u, float* __restrict__ part,
+                                                  int rows_per_blk, const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int c0 = blockIdx.y * (T * CPT);
+    float cs[CPT];
+    bool ok[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        ok[k] = c0 + k * T + static_cast<int>(threadIdx.x) < n;
+        cs[k] = 0.0f;
+    }
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    for (int i = r0; i < r1; ++i) {
+        const float* row = C + static_cast<int64_t>(i) * ldc;
+        const float ui = u[i];
+        float c[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) c[k] = ok[k] ? ld_stream(row + c0 + k * T + threadIdx.x) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+            if (ok[k]) cs[k] = fmaf(ui, exp2_fast(c[k] * scale2), cs[k]);
+    }
+    float* out = part + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+        if (ok[k]) out[c0 + k * T + threadIdx.x] = cs[k];
+}
+
+// Row sweep: one wave per row, u_i = blend(u_i, 1 / (sum_j K_ij v_j + eps)).
+template <int W, bool VEC>
+__global__ __launch_bounds__(W * kWave) void std_row_sweep(const float* __restrict__ C, int n, int64_t ldc,
+                                                          float scale2, const float* __restrict__ v,
+                                                          float* __restrict__ u, const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const float mom = ctl->momentum;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * W + threadIdx.x / kWave;
+    if (i >= n) return;
+    const float* row = C + i * ldc;
+    float s = 0.0f;
+    if (VEC) {
+        for (int j = lane * 4; j < n; j += kWave * 4) {
+            const float4 c = ld_stream4(row + j);
+            const float4 a = *reinterpret_cast<const float4*>(v + j);
+            s += (exp2_fast(c.x * scale2) * a.x + exp2_fast(c.y * scale2) * a.y) +
+                 (exp2_fast(c.z * scale2) * a.z + exp2_fast(c.w * scale2) * a.w);
+        }
+    } else {
+        for (int j = lane; j < n; j += kWave) s = fmaf(exp2_fast(row[j] * scale2), v[j], s);
+    }
+    s = wave_sum(s);
+    if (lane == 0) u[i] = std_blend(u[i], 1.0f / (s + kStdEps), mom);
+}
+
+// Combine: column sums S_j = sum_i u_i K_ij (FP64 over the partials) give
+// the raw next v_j = 1 / (S_j + eps) and the column marginal v_j S_j of the
+// current (u, v).
+__global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
+    const float* __restrict__ part, int nblk, int n, const float* __restrict__ v_in, float* __restrict__ v_out,
+    int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
+    double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
+    __shared__ double rs[kCombWaves][kCombCols];
+    if (ctl->done) return;
+    const int lane = threadIdx.x & (kCombCols - 1);
+    const int wave = threadIdx.x / kCombCols;
+    const int j = blockIdx.x * kCombCols + lane;
+    double sum = 0.0;
+    if (j < n)
+        for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
+    rs[wave][lane] = sum;
+    __syncthreads();
+    double err = 0.0;
+    if (wave == 0 && j < n) {
+        double S = 0.0;
+        for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+        v_out[j] = static_cast<float>(1.0 / (S + static_cast<double>(kStdEps)));
+        if (want_err) err = fabs(static_cast<double>(v_in[j]) * S - 1.0);
+    }
+    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl);
+}
+
+// v_next = v^(1-m) * raw^m once the check of this combine has set m.
+__global__ void std_apply_momentum(const float* __restrict__ v_in, float* __restrict__ v_io, int n,
+                                   const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    v_io[j] = std_blend(v_in[j], v_io[j], ctl->momentum);
+}
+
+__global__ void std_init(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ uw,
+                         float* __restrict__ vw, int n, SinkCtl* __restrict__ ctl) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        uw[j] = u[j];
+        vw[j] = v[j];
+    }
+    if (j == 0) {
+        ctl->done = 0;
+        ctl->iterations = 0;
+        ctl->final_buf = 0;
+        ctl->nerr = 0;
+        ctl->ticket = 0u;
+        ctl->momentum = 1.0f;
+    }
+}
+
+__global__ void std_finish(const float* __restrict__ uw, const float* __restrict__ va, const float* __restrict__ vb,
+                           float* __restrict__ u, float* __restrict__ v, int n, const SinkCtl* __restrict__ ctl,
+                           int default_buf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int buf = ctl->done ? ctl->final_buf : default_buf;
+    u[j] = uw[j];
+    v[j] = buf ? vb[j] : va[j];
 }
 
 // Row marginal error of a materialised kernel, cuda_kernels/sinkhorn_log.cu:68-98:
@@ -587,6 +822,64 @@ int launch_row(const float* C, int n, int64_t ldc, float cscale, const float* ad
         hipLaunchKernelGGL((sk_row_sweep<kRowWaves, false>), grid, dim3(kRowWaves * kWave), 0, s, C, n, ldc, cscale,
                            add, ascale, oscale, out, mom, ctl);
     return check_launch("sk_row_sweep");
+}
+
+template <int T, int G>
+int launch_std_fused_tg(const float* C, int n, float scale2, const float* v, float* u, float* part, const Plan& p,
+                        const SinkCtl* ctl, hipStream_t s) {
+    const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float) * 2 * (T / kWave);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
+        attr_set = true;
+    }
+    if (lds > static_cast<size_t>(max_lds_bytes()))
+        return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
+                    max_lds_bytes());
+    hipLaunchKernelGGL((std_fused_sweep<T, G>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v, u, part,
+                       p.rows_per_blk, ctl);
+    return check_launch("std_fused_sweep");
+}
+
+int launch_std_fused(const float* C, int n, float scale2, const float* v, float* u, float* part, const Plan& p,
+                     const SinkCtl* ctl, hipStream_t s) {
+    if (p.threads == 256) {
+        switch (p.groups) {
+            case 1: return launch_std_fused_tg<256, 1>(C, n, scale2, v, u, part, p, ctl, s);
+            case 2: return launch_std_fused_tg<256, 2>(C, n, scale2, v, u, part, p, ctl, s);
+            case 4: return launch_std_fused_tg<256, 4>(C, n, scale2, v, u, part, p, ctl, s);
+            case 6: return launch_std_fused_tg<256, 6>(C, n, scale2, v, u, part, p, ctl, s);
+            default: return launch_std_fused_tg<256, 8>(C, n, scale2, v, u, part, p, ctl, s);
+        }
+    }
+    switch (p.groups) {
+        case 1: return launch_std_fused_tg<1024, 1>(C, n, scale2, v, u, part, p, ctl, s);
+        case 2: return launch_std_fused_tg<1024, 2>(C, n, scale2, v, u, part, p, ctl, s);
+        case 4: return launch_std_fused_tg<1024, 4>(C, n, scale2, v, u, part, p, ctl, s);
+        case 6: return launch_std_fused_tg<1024, 6>(C, n, scale2, v, u, part, p, ctl, s);
+        default: return launch_std_fused_tg<1024, 8>(C, n, scale2, v, u, part, p, ctl, s);
+    }
+}
+
+int launch_std_col(const float* C, int n, float scale2, const float* u, float* part, const Plan& p,
+                   const SinkCtl* ctl, hipStream_t s) {
+    hipLaunchKernelGGL((std_col_sweep<kColThreads, kColCPT>), dim3(p.col_nblk, p.col_chunks), dim3(kColThreads), 0,
+                       s, C, n, static_cast<int64_t>(n), scale2, u, part, p.col_rows, ctl);
+    return check_launch("std_col_sweep");
+}
+
+int launch_std_row(const float* C, int n, float scale2, const float* v, float* u, const SinkCtl* ctl,
+                   hipStream_t s) {
+    const bool vec = (n % 4 == 0) && aligned16(C) && aligned16(v);
+    const dim3 grid(static_cast<unsigned>(div_up(n, kRowWaves)));
+    if (vec)
+        hipLaunchKernelGGL((std_row_sweep<kRowWaves, true>), grid, dim3(kRowWaves * kWave), 0, s, C, n,
+                           static_cast<int64_t>(n), scale2, v, u, ctl);
+    else
+        hipLaunchKernelGGL((std_row_sweep<kRowWaves, false>), grid, dim3(kRowWaves * kWave), 0, s, C, n,
+                           static_cast<int64_t>(n), scale2, v, u, ctl);
+    return check_launch("std_row_sweep");
 }
 
 // Pinned host mirror of the control block, grown on demand.
@@ -887,6 +1180,119 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     if (errors) std::memcpy(errors, herr, sizeof(double) * static_cast<size_t>(std::max(0, ne)));
     if (error_count) *error_count = hc->nerr;
     if (int rc = timer_total(sweep_ms, sweep_launches)) return rc;
+    return CG_OK;
+}
+
+int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, float* u, float* v,
+                               int max_iterations, double threshold, int eval_freq, int momentum_start, int fused,
+                               double* errors, int errors_capacity, int* error_count, int* iterations,
+                               void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!C || !u || !v || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_sinkhorn_standard_solve: bad arguments (n=%lld)", static_cast<long long>(n));
+    if (!(regularization > 0.0f)) return fail(CG_EINVAL, "sinkhorn regularization must be > 0");
+    if (eval_freq <= 0) return fail(CG_EINVAL, "sinkhorn eval_freq must be > 0");
+    if (max_iterations < 0) return fail(CG_EINVAL, "sinkhorn iterations must be >= 0");
+    const int ni = static_cast<int>(n);
+    const int use_fused = fused && cg_sinkhorn_fused_supported(n) && aligned16(C) && n % 4 == 0;
+    const Plan p = make_plan(n, use_fused);
+    // Checks at 0, f, 2f, ...: at most max_iterations / f + 1 <= capacity.
+    const int cap = err_capacity(max_iterations, eval_freq);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    SinkWs w{};
+    carve(cv, n, p, cap, &w);
+    if (!workspace || !cv.ok())
+        return fail(CG_EWORKSPACE, "cg_sinkhorn_standard_solve: workspace too small (%zu < %zu)", workspace_bytes,
+                    cv.used);
+    if (int rc = ensure_pinned(ctl_bytes(cap))) return rc;
+    const float scale2 = -kLog2e / regularization;
+    const bool momentum = momentum_start >= 0;
+    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
+    float* part = static_cast<float*>(w.part);
+    float* uw = w.u2;  // natural units here
+    hipLaunchKernelGGL(std_init, vgrid, dim3(256), 0, s, u, v, uw, w.v2[0], ni, w.ctl);
+    if (int rc = check_launch("std_init")) return rc;
+
+    struct Pending {
+        hipEvent_t ev;
+        int slot;
+    };
+    std::vector<Pending> pending;
+    int next_slot = 0;
+    bool stopped = false;
+    auto post_check = [&]() -> int {
+        hipEvent_t ev;
+        CG_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const int slot = next_slot++ & 63;
+        CG_CHECK_HIP(hipMemcpyAsync(g_done_slots + slot, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        CG_CHECK_HIP(hipEventRecord(ev, s));
+        pending.push_back({ev, slot});
+        if (pending.size() >= 2) {
+            Pending old = pending.front();
+            pending.erase(pending.begin());
+            CG_CHECK_HIP(hipEventSynchronize(old.ev));
+            CG_CHECK_HIP(hipEventDestroy(old.ev));
+            if (g_done_slots[old.slot]) stopped = true;
+        }
+        return 0;
+    };
+    auto combine = [&](int nblk, int cur, int want_err, int check_index) -> int {
+        hipLaunchKernelGGL(std_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, part, nblk, ni,
+                           w.v2[cur], w.v2[cur ^ 1], want_err, check_index, cur, threshold, eval_freq, momentum_start,
+                           w.errpart, w.ctl);
+        if (int rc = check_launch("std_combine")) return rc;
+        if (momentum) {
+            hipLaunchKernelGGL(std_apply_momentum, vgrid, dim3(256), 0, s, w.v2[cur], w.v2[cur ^ 1], ni, w.ctl);
+            if (int rc = check_launch("std_apply_momentum")) return rc;
+        }
+        return 0;
+    };
+    // The reference checks at iteration 0 too (cugal/sinkhorn.py:138).
+    auto is_check = [&](int it) { return it % eval_freq == 0; };
+
+    int cur = 0;
+    int final_default = 0;
+    if (max_iterations > 0) {
+        // v_1 from the incoming u, by the column sweep.
+        if (int rc = launch_std_col(C, ni, scale2, uw, part, p, w.ctl, s)) return rc;
+        if (int rc = combine(p.col_nblk, 0, 0, -1)) return rc;
+        cur = 1;
+        if (use_fused) {
+            for (int it = 0; it < max_iterations && !stopped; ++it) {
+                const bool chk = is_check(it);
+                if (int rc = launch_std_fused(C, ni, scale2, w.v2[cur], uw, part, p, w.ctl, s)) return rc;
+                if (int rc = combine(p.nblk, cur, chk ? 1 : 0, it)) return rc;
+                final_default = cur;
+                cur ^= 1;
+                if (chk)
+                    if (int rc = post_check()) return rc;
+            }
+        } else {
+            // Row sweep (u from v), then the column sweep of the new u, which
+            // both checks this iteration and forms the next v.
+            for (int it = 0; it < max_iterations && !stopped; ++it) {
+                const bool chk = is_check(it);
+                if (int rc = launch_std_row(C, ni, scale2, w.v2[cur], uw, w.ctl, s)) return rc;
+                if (int rc = launch_std_col(C, ni, scale2, uw, part, p, w.ctl, s)) return rc;
+                if (int rc = combine(p.col_nblk, cur, chk ? 1 : 0, it)) return rc;
+                final_default = cur;
+                cur ^= 1;
+                if (chk)
+                    if (int rc = post_check()) return rc;
+            }
+        }
+    }
+    hipLaunchKernelGGL(std_finish, vgrid, dim3(256), 0, s, uw, w.v2[0], w.v2[1], u, v, ni, w.ctl, final_default);
+    if (int rc = check_launch("std_finish")) return rc;
+    CG_CHECK_HIP(hipMemcpyAsync(g_mirror.ptr, w.ctl, ctl_bytes(cap), hipMemcpyDeviceToHost, s));
+    CG_CHECK_HIP(hipStreamSynchronize(s));
+    for (auto& pe : pending) (void)hipEventDestroy(pe.ev);
+    const SinkCtl* hc = static_cast<const SinkCtl*>(g_mirror.ptr);
+    const double* herr = reinterpret_cast<const double*>(static_cast<const char*>(g_mirror.ptr) + sizeof(SinkCtl));
+    if (iterations) *iterations = hc->done ? hc->iterations : max_iterations;
+    const int ne = std::min(hc->nerr, errors ? errors_capacity : 0);
+    if (errors) std::memcpy(errors, herr, sizeof(double) * static_cast<size_t>(std::max(0, ne)));
+    if (error_count) *error_count = hc->nerr;
     return CG_OK;
 }
 

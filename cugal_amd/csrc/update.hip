@@ -1,6 +1,9 @@
 // Frank-Wolfe step of the quasi-permutation, cugal/pred.py:77-86 + :133-141
 // (fused form of cuda_kernels/update_quasi_permutation.cu:7-35):
-//   plan = exp(-C/reg + u_i + v_j),  gap = sum (plan - P) C,  P += a (plan - P)
+//   plan = exp(-C/reg + u_i + v_j)          (LOG duals), or
+//   plan = u_i exp(-C/reg) v_j              (STANDARD scalings,
+//                                            cugal/sinkhorn.py:20-23),
+//   gap = sum (plan - P) C,  P += a (plan - P)
 // One read of C, one read and one write of P: 12 n^2 bytes.  The gap is
 // accumulated in FP64 and reduced in a fixed order (deterministic).
 #include "common.hpp"
@@ -12,7 +15,13 @@ namespace {
 constexpr int kUpdThreads = 256;
 constexpr int kUpdMaxBlocks = 2048;
 
-template <bool VEC>
+template <bool STD>
+__device__ __forceinline__ float plan_of(float c, float neg_rinv, float ui, float vj) {
+    if constexpr (STD) return ui * exp2_fast(c * neg_rinv * kLog2e) * vj;
+    return exp2_fast((fmaf(c, neg_rinv, ui) + vj) * kLog2e);
+}
+
+template <bool VEC, bool STD>
 __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, const float* __restrict__ C, int n,
                                                         const float* __restrict__ u, const float* __restrict__ v,
                                                         float alpha, float neg_rinv, int rows_per_blk,
@@ -30,10 +39,10 @@ __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, 
                 const float4 c = ld_stream4(C + base + j);
                 const float4 vv = *reinterpret_cast<const float4*>(v + j);
                 float4 p = *reinterpret_cast<const float4*>(P + base + j);
-                const float e0 = exp2_fast((fmaf(c.x, neg_rinv, ui) + vv.x) * kLog2e) - p.x;
-                const float e1 = exp2_fast((fmaf(c.y, neg_rinv, ui) + vv.y) * kLog2e) - p.y;
-                const float e2 = exp2_fast((fmaf(c.z, neg_rinv, ui) + vv.z) * kLog2e) - p.z;
-                const float e3 = exp2_fast((fmaf(c.w, neg_rinv, ui) + vv.w) * kLog2e) - p.w;
+                const float e0 = plan_of<STD>(c.x, neg_rinv, ui, vv.x) - p.x;
+                const float e1 = plan_of<STD>(c.y, neg_rinv, ui, vv.y) - p.y;
+                const float e2 = plan_of<STD>(c.z, neg_rinv, ui, vv.z) - p.z;
+                const float e3 = plan_of<STD>(c.w, neg_rinv, ui, vv.w) - p.w;
                 rowgap += (e0 * c.x + e1 * c.y) + (e2 * c.z + e3 * c.w);
                 p.x = fmaf(alpha, e0, p.x);
                 p.y = fmaf(alpha, e1, p.y);
@@ -45,7 +54,7 @@ __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, 
             for (int j = threadIdx.x; j < n; j += kUpdThreads) {
                 const float c = C[base + j];
                 const float p = P[base + j];
-                const float d = exp2_fast((fmaf(c, neg_rinv, ui) + v[j]) * kLog2e) - p;
+                const float d = plan_of<STD>(c, neg_rinv, ui, v[j]) - p;
                 rowgap += d * c;
                 P[base + j] = fmaf(alpha, d, p);
             }
@@ -83,29 +92,52 @@ size_t cg_update_workspace(int64_t n) {
     return sizeof(double) * static_cast<size_t>(update_blocks(n, &rpb)) + 256;
 }
 
-int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
-                                    float alpha, float regularization, double* gap_device, void* workspace,
-                                    size_t workspace_bytes, cg_stream_t stream) {
+namespace {
+
+int run_update(bool standard, float* P, const float* C, int64_t n, const float* u, const float* v, float alpha,
+               float regularization, double* gap_device, void* workspace, size_t workspace_bytes,
+               hipStream_t s, const char* who) {
     if (!P || !C || !u || !v || !gap_device || n <= 0 || n > (int64_t(1) << 30))
-        return fail(CG_EINVAL, "cg_update_quasi_permutation_log: bad arguments");
+        return fail(CG_EINVAL, "%s: bad arguments", who);
     if (!(regularization > 0.0f)) return fail(CG_EINVAL, "regularization must be > 0");
-    if (!workspace || workspace_bytes < cg_update_workspace(n))
-        return fail(CG_EWORKSPACE, "cg_update_quasi_permutation_log: workspace too small");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!workspace || workspace_bytes < cg_update_workspace(n)) return fail(CG_EWORKSPACE, "%s: workspace too small", who);
     int rpb = 0;
     const int blocks = update_blocks(n, &rpb);
     double* part = static_cast<double*>(workspace);
     const bool vec = (n % 4 == 0) && aligned16(P) && aligned16(C) && aligned16(v);
     const float neg_rinv = -1.0f / regularization;
-    if (vec)
-        hipLaunchKernelGGL(fw_update<true>, dim3(blocks), dim3(kUpdThreads), 0, s, P, C, static_cast<int>(n), u, v,
-                           alpha, neg_rinv, rpb, part);
+    const int ni = static_cast<int>(n);
+    if (vec && standard)
+        hipLaunchKernelGGL((fw_update<true, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+                           neg_rinv, rpb, part);
+    else if (vec)
+        hipLaunchKernelGGL((fw_update<true, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+                           neg_rinv, rpb, part);
+    else if (standard)
+        hipLaunchKernelGGL((fw_update<false, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+                           neg_rinv, rpb, part);
     else
-        hipLaunchKernelGGL(fw_update<false>, dim3(blocks), dim3(kUpdThreads), 0, s, P, C, static_cast<int>(n), u, v,
-                           alpha, neg_rinv, rpb, part);
+        hipLaunchKernelGGL((fw_update<false, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+                           neg_rinv, rpb, part);
     if (int rc = check_launch("fw_update")) return rc;
     hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, part, blocks, gap_device);
     return check_launch("sum_parts");
+}
+
+}  // namespace
+
+int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
+                                    float alpha, float regularization, double* gap_device, void* workspace,
+                                    size_t workspace_bytes, cg_stream_t stream) {
+    return run_update(false, P, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
+                      reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_log");
+}
+
+int cg_update_quasi_permutation_standard(float* P, const float* C, int64_t n, const float* u, const float* v,
+                                         float alpha, float regularization, double* gap_device, void* workspace,
+                                         size_t workspace_bytes, cg_stream_t stream) {
+    return run_update(true, P, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
+                      reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_standard");
 }
 
 }  // extern "C"

@@ -26,7 +26,7 @@ from .feature_extraction import Features
 from .greedy_lap import greedy_lap
 from .hungarian import dense_hungarian
 from .profile import Phase, Profile, SinkhornProfile, TimeStamp
-from .sinkhorn import LogKernel, SinkhornState
+from .sinkhorn import LogKernel, SinkhornState, StandardKernel
 
 
 def _check_config(config: Config) -> None:
@@ -37,8 +37,10 @@ def _check_config(config: Config) -> None:
         )
     if config.dtype != torch.float32:
         raise NotImplementedError("the HIP kernels compute in FP32: set Config(dtype=torch.float32)")
-    if config.sinkhorn_method != SinkhornMethod.LOG:
-        raise NotImplementedError("the HIP Sinkhorn is the log-domain one: set sinkhorn_method=SinkhornMethod.LOG")
+    if config.sinkhorn_method not in (SinkhornMethod.LOG, SinkhornMethod.STANDARD):
+        raise NotImplementedError(
+            "SinkhornMethod.MIX has no solver in the reference (cugal/sinkhorn.py:146-156): use LOG or STANDARD"
+        )
     if config.gradient_precision not in ("fp32", "bf16x2"):
         raise NotImplementedError(f"gradient_precision {config.gradient_precision!r} is not available")
 
@@ -127,14 +129,35 @@ def update_quasi_permutation_log(P: torch.Tensor, K: LogKernel, u: torch.Tensor,
     return abs(float(out.item())) if gap is None else None
 
 
+def update_quasi_permutation_standard(P: torch.Tensor, K: StandardKernel, u: torch.Tensor, v: torch.Tensor,
+                                      alpha: float, gap: torch.Tensor | None = None):
+    """P += alpha (diag(u) K diag(v) - P) in one pass with K = exp(-C/reg)
+    (cugal/pred.py:77-86 with scale_kernel_matrix); the gap as in
+    update_quasi_permutation_log."""
+    n = P.shape[0]
+    dev = P.device
+    out = torch.empty(1, dtype=torch.float64, device=dev) if gap is None else gap
+    ws = nat.workspace(dev, nat.query("cg_update_workspace", n), "update")
+    nat.call(
+        "cg_update_quasi_permutation_standard", nat.ptr(P), nat.ptr(K.cost), n, nat.ptr(u), nat.ptr(v),
+        float(alpha), float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    return abs(float(out.item())) if gap is None else None
+
+
 def update_quasi_permutation(P: torch.Tensor, K, u: torch.Tensor, v: torch.Tensor, alpha: float,
                              sinkhorn_method: SinkhornMethod) -> torch.Tensor:
     """Reference-compatible form (cugal/pred.py:77-86): updates P and returns
     the step alpha (plan - P)."""
-    if sinkhorn_method != SinkhornMethod.LOG:
-        raise NotImplementedError("only the log-domain update is implemented")
     before = P.clone()
-    update_quasi_permutation_log(P, K, u, v, alpha, gap=torch.empty(1, dtype=torch.float64, device=P.device))
+    gap = torch.empty(1, dtype=torch.float64, device=P.device)
+    match sinkhorn_method:
+        case SinkhornMethod.LOG:
+            update_quasi_permutation_log(P, K, u, v, alpha, gap=gap)
+        case SinkhornMethod.STANDARD:
+            update_quasi_permutation_standard(P, K, u, v, alpha, gap=gap)
+        case _:
+            raise NotImplementedError("SinkhornMethod.MIX has no solver in the reference")
     return P - before
 
 
@@ -179,7 +202,10 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
             profile.log_time(start_time, Phase.SINKHORN)
 
             alpha = 2.0 / float(2.0 + it)
-            update_quasi_permutation_log(P, K, u, v, alpha, gap=gap)
+            if config.sinkhorn_method == SinkhornMethod.STANDARD:
+                update_quasi_permutation_standard(P, K, u, v, alpha, gap=gap)
+            else:
+                update_quasi_permutation_log(P, K, u, v, alpha, gap=gap)
 
             profile.frank_wolfe_iterations += 1
             if config.frank_wolfe_threshold is not None and abs(float(gap.item())) < config.frank_wolfe_threshold:

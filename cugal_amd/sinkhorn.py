@@ -3,11 +3,12 @@ cugal/sinkhorn.py:13-156.
 
 ``SinkhornState`` keeps the dual potentials u, v on the device; they
 warm-start every solve of an alignment (cugal/pred.py:95).  The LOG method
-runs entirely in ``cg_sinkhorn_log_solve``: the iterations, the column-marginal
-checks every ``sinkhorn_eval_freq`` iterations, the threshold stop and Lehmann
+runs entirely in ``cg_sinkhorn_log_solve`` and STANDARD in
+``cg_sinkhorn_standard_solve``: the iterations, the column-marginal checks
+every ``sinkhorn_eval_freq`` iterations, the threshold stop and Lehmann
 momentum are all decided on the device, and the host synchronises once per
-solve.  K = -C / reg is never materialised; ``solve`` returns it as a
-:class:`LogKernel` view of the cost.
+solve.  K (-C / reg, or exp(-C / reg)) is never materialised; ``solve``
+returns it as a :class:`LogKernel` / :class:`StandardKernel` view of the cost.
 """
 
 from __future__ import annotations
@@ -41,8 +42,22 @@ class LogKernel:
         return -self.cost / self.regularization
 
 
-def scale_kernel_matrix(K: torch.Tensor, u: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
-    """diag(u) K diag(v), in place (cugal/sinkhorn.py:20-23)."""
+@dataclass
+class StandardKernel:
+    """K = exp(-cost / regularization), kept implicit."""
+
+    cost: torch.Tensor
+    regularization: float
+
+    def materialize(self) -> torch.Tensor:
+        return torch.exp(self.cost / -self.regularization)
+
+
+def scale_kernel_matrix(K, u: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
+    """diag(u) K diag(v), in place on a tensor K (cugal/sinkhorn.py:20-23);
+    a StandardKernel is materialised first."""
+    if isinstance(K, StandardKernel):
+        K = K.materialize()
     K *= v.reshape(1, -1)
     K *= u.reshape(-1, 1)
     return K
@@ -133,10 +148,25 @@ class SinkhornState:
         return LogKernel(C, config.sinkhorn_regularization), self.u, self.v
 
     def solve_standard(self, C: torch.Tensor, config: Config, profile: SinkhornProfile | None = None):
-        raise NotImplementedError(
-            "STANDARD Sinkhorn (cugal/sinkhorn.py:122-144) is not implemented on the ROCm path; "
-            "use sinkhorn_method=SinkhornMethod.LOG with dtype torch.float32"
+        """Scaling-form solve of cugal/sinkhorn.py:122-144 on the device."""
+        profile = SinkhornProfile() if profile is None else profile
+        if not can_use_cuda(config):
+            raise NotImplementedError("the HIP Sinkhorn needs config.device 'cuda' and dtype torch.float32")
+        C = C.contiguous()
+        n = C.shape[0]
+        nat.require_device_tensor(C, "cost", torch.float32, n * n)
+        nat.require_device_tensor(self.u, "u", torch.float32, n)
+        nat.require_device_tensor(self.v, "v", torch.float32, n)
+        start_time = TimeStamp(config.device)
+        errors, iterations = solve_standard_native(
+            C, config.sinkhorn_regularization, self.u, self.v, config.sinkhorn_iterations,
+            config.sinkhorn_threshold, config.sinkhorn_eval_freq, config.sinkhorn_momentum_start,
+            fused=config.sinkhorn_fused,
         )
+        profile.iteration_count = iterations
+        profile.time = TimeStamp(config.device).elapsed_seconds(start_time)
+        profile.errors = errors
+        return StandardKernel(C, config.sinkhorn_regularization), self.u, self.v
 
     def solve(self, C: torch.Tensor, config: Config, profile: SinkhornProfile | None = None):
         match config.sinkhorn_method:
@@ -192,3 +222,27 @@ def solve_log_native(
         SweepTiming.launches += int(launches[0])
     count = int(min(err_count[0], cap))
     return [float(e) for e in errors[:count]], int(iters[0]), float(ms[0])
+
+
+def solve_standard_native(
+    C: torch.Tensor, regularization: float, u: torch.Tensor, v: torch.Tensor, max_iterations: int,
+    threshold: float, eval_freq: int, momentum_start, fused: bool = True,
+) -> tuple[list[float], int]:
+    """Run cg_sinkhorn_standard_solve in place on (u, v).  Returns (errors,
+    iteration count)."""
+    n = C.shape[0]
+    dev = C.device
+    ws = nat.workspace(dev, nat.query("cg_sinkhorn_workspace", n, int(fused), max_iterations, eval_freq), "sinkhorn")
+    cap = max_iterations // max(1, eval_freq) + 4
+    errors = np.zeros(cap, dtype=np.float64)
+    err_count = np.zeros(1, dtype=np.int32)
+    iters = np.zeros(1, dtype=np.int32)
+    nat.call(
+        "cg_sinkhorn_standard_solve", nat.ptr(C), n, float(regularization), nat.ptr(u), nat.ptr(v),
+        int(max_iterations), float(threshold), int(eval_freq),
+        -1 if momentum_start is None else int(momentum_start), int(bool(fused)),
+        errors.ctypes.data, cap, err_count.ctypes.data, iters.ctypes.data,
+        nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    count = int(min(err_count[0], cap))
+    return [float(e) for e in errors[:count]], int(iters[0])

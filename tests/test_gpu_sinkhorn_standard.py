@@ -1,0 +1,151 @@
+"""HIP STANDARD Sinkhorn (cg_sinkhorn_standard_solve) and its Frank-Wolfe
+update against the FP64 oracle of cugal/sinkhorn.py:122-144 and
+cugal/pred.py:77-86, through the C ABI."""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from cugal_amd import Config, HungarianMethod, SinkhornMethod, cugal
+from cugal_amd.pred import update_quasi_permutation_standard
+from cugal_amd.profile import Profile
+from cugal_amd.sinkhorn import StandardKernel, solve_standard_native, solve_log_native
+from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi
+from oracle import fugal as oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cost(n, seed, scale=5.0):
+    g = torch.Generator().manual_seed(seed)
+    return torch.rand((n, n), generator=g, dtype=torch.float64) * scale
+
+
+def _run_gpu(C64, reg, iters, thr, freq, fused, device, momentum_start=None):
+    n = C64.shape[0]
+    C = C64.to(torch.float32).to(device).contiguous()
+    u = torch.full((n,), 1.0 / n, device=device)
+    v = torch.full((n,), 1.0 / n, device=device)
+    errors, count = solve_standard_native(C, reg, u, v, iters, thr, freq, momentum_start, fused=fused)
+    torch.cuda.synchronize()
+    return u.double().cpu(), v.double().cpu(), errors, count
+
+
+def _run_oracle(C64, reg, iters, thr, freq, momentum_start=None):
+    st = oracle.SinkhornState(C64.shape[0], "STANDARD")
+    return st.solve_standard(C64.to(torch.float32).double(), reg, iters, thr, freq, momentum_start)
+
+
+# FP32 scalings against FP64: each u_i (v_j) is one over an FP32 sum of n
+# positive terms (relative rounding ~ n^(1/2) 2^-24 per step, damped by the
+# iteration); 2e-4 relative on the scalings and 5e-4 on plan entries.
+S_RTOL = 2e-4
+PLAN_RTOL = 5e-4
+
+
+def std_error_floor(n):
+    """FP32 floor of sum_j |v_j sum_i u_i K_ij - 1|: n columns, each a
+    product of FP32 quantities carrying ~8 ulps."""
+    return n * 8 * 2.0 ** -24
+
+
+@pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
+@pytest.mark.parametrize("fused", [True, False])
+def test_fixed_iterations_match_oracle(device, n, fused):
+    C = _cost(n, seed=n)
+    reg = 1.0
+    u, v, errors, count = _run_gpu(C, reg, 40, 0.0, 10, fused, device)
+    K, ru, rv, res = _run_oracle(C, reg, 40, 0.0, 10)
+    assert count == res.iteration_count == 40
+    np.testing.assert_allclose(u.numpy(), ru.numpy(), rtol=S_RTOL)
+    np.testing.assert_allclose(v.numpy(), rv.numpy(), rtol=S_RTOL)
+    np.testing.assert_allclose(oracle.plan_standard(K, u, v).numpy(), oracle.plan_standard(K, ru, rv).numpy(),
+                               rtol=PLAN_RTOL)
+    # errors at iterations 0, 10, 20, 30 (the reference checks iteration 0)
+    assert len(errors) == len(res.errors) == 4
+    np.testing.assert_allclose(errors, res.errors, rtol=2e-3, atol=std_error_floor(n))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_threshold_stop_matches_oracle(device, fused):
+    n = 512
+    C = _cost(n, seed=3, scale=8.0)
+    reg, thr = 0.7, 1e-2
+    u, v, errors, count = _run_gpu(C, reg, 500, thr, 10, fused, device)
+    _, ru, rv, res = _run_oracle(C, reg, 500, thr, 10)
+    assert res.iteration_count < 500
+    assert count == res.iteration_count
+    assert errors[-1] < thr <= errors[-2]
+    np.testing.assert_allclose(u.numpy(), ru.numpy(), rtol=1e-3)
+    np.testing.assert_allclose(v.numpy(), rv.numpy(), rtol=1e-3)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_momentum_matches_oracle(device, fused):
+    n = 300 if not fused else 320
+    C = _cost(n, seed=11, scale=6.0)
+    reg = 0.8
+    u, v, errors, count = _run_gpu(C, reg, 60, 0.0, 10, fused, device, momentum_start=20)
+    _, ru, rv, res = _run_oracle(C, reg, 60, 0.0, 10, momentum_start=20)
+    np.testing.assert_allclose(u.numpy(), ru.numpy(), rtol=1e-3)
+    np.testing.assert_allclose(v.numpy(), rv.numpy(), rtol=1e-3)
+    np.testing.assert_allclose(errors, res.errors, rtol=1e-2, atol=std_error_floor(n))
+
+
+def test_standard_and_log_reach_the_same_plan(device):
+    # Both methods iterate towards the same doubly stochastic scaling.
+    n = 256
+    C = _cost(n, seed=5).to(torch.float32).to(device)
+    reg = 1.0
+    us = torch.full((n,), 1.0 / n, device=device)
+    vs = torch.full((n,), 1.0 / n, device=device)
+    solve_standard_native(C, reg, us, vs, 300, 0.0, 10, None)
+    ul = torch.zeros(n, device=device)
+    vl = torch.zeros(n, device=device)
+    solve_log_native(C, reg, ul, vl, 300, 0.0, 10, None)
+    K = torch.exp(-C.double() / reg)
+    plan_s = K * us.double()[:, None] * vs.double()[None, :]
+    plan_l = torch.exp(-C.double() / reg + ul.double()[:, None] + vl.double()[None, :])
+    np.testing.assert_allclose(plan_s.cpu().numpy(), plan_l.cpu().numpy(), rtol=1e-3, atol=1e-9)
+    np.testing.assert_allclose(plan_s.sum(0).cpu().numpy(), 1.0, atol=1e-4)
+
+
+def test_standard_update_matches_oracle(device):
+    n = 777
+    g = torch.Generator().manual_seed(2)
+    C = torch.rand((n, n), generator=g) * 5
+    u = torch.rand(n, generator=g) / n + 0.5 / n
+    v = torch.rand(n, generator=g) + 0.5
+    P = torch.rand((n, n), generator=g) / n
+    reg, alpha = 0.9, 0.4
+    Pd = P.to(device)
+    gap = torch.empty(1, dtype=torch.float64, device=device)
+    update_quasi_permutation_standard(Pd, StandardKernel(C.to(device), reg), u.to(device), v.to(device), alpha,
+                                      gap=gap)
+    K = torch.exp(C.double() / -reg)
+    Pr = P.double().clone()
+    step = oracle.update_quasi_permutation(Pr, oracle.plan_standard(K, u.double(), v.double()), alpha)
+    want_gap = oracle.duality_gap(C.double(), step, alpha)
+    np.testing.assert_allclose(Pd.double().cpu().numpy(), Pr.numpy(), rtol=1e-5, atol=1e-9)
+    assert abs(abs(float(gap.item())) - want_gap) <= 1e-4 * want_gap + 1e-6
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+def test_pipeline_standard_matches_oracle(device, sparse):
+    inst = erdos_renyi(400, 0.02, 0.05, seed=0)
+    gpu = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.STANDARD,
+                 sinkhorn_regularization=1.0, mu=0.5, iter_count=5, frank_wolfe_iter_count=5,
+                 use_sparse_adjacency=sparse, hungarian_method=HungarianMethod.DOUBLE_GREEDY)
+    ref = oracle.OracleConfig(sinkhorn_regularization=1.0, sinkhorn_method="STANDARD", mu=0.5, iter_count=5,
+                              frank_wolfe_iter_count=5, use_sparse_adjacency=sparse)
+    profile = Profile()
+    _, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, profile)
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)
+    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    ref_acc = alignment_accuracy(inst.source_mapping, answer_columns(run.mapping))
+    print(f"\nSTANDARD pipeline sparse={sparse}: accuracy {acc:.4f} vs oracle {ref_acc:.4f}")
+    assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
+    got_iters = [p.iteration_count for p in profile.sinkhorn_profiles]
+    assert all(abs(a - b) <= 10 for a, b in zip(got_iters, run.sinkhorn_iterations))
