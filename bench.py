@@ -14,7 +14,16 @@ Reported: `value` = seconds per alignment on the fused path ("mixed": the
 cost is read once per Sinkhorn iteration), beside the plain two-sweep path
 on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
 4 n^2 per sweep over the HIP-event time of the sweep launches) and the FP64
-CPU oracle timed on a bounded sample of the same Sinkhorn work.
+CPU oracle timed on a bounded sample of the same Sinkhorn work.  The
+precision form (gradient on the matrix cores: 0/1 BF16 adjacency times BF16
+hi + lo P) runs once on the same input (`precision_form`: time, accuracy,
+matrix-core rate, and the gradient's error against FP64 on sampled rows).
+
+Other workloads (`--workload`): `ba8192` (Barabasi-Albert n=8192, m=5, 2%
+edge noise; the LAP as greedy and as the exact device Hungarian, checked
+against scipy on the same P) and `sbm65536` (stochastic block model, 64
+blocks of 1024, intra/inter degrees 48/16, 1% noise, 15 lambda stages of one
+Frank-Wolfe step, greedy LAP; P is 17 GB).
 """
 
 from __future__ import annotations
@@ -36,7 +45,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_config(fused: bool, args):
+def make_config(fused: bool, args, precision: str = "fp32", lap: str = "DOUBLE_GREEDY"):
     import torch
 
     from cugal_amd import Config, HungarianMethod, SinkhornMethod
@@ -46,7 +55,7 @@ def make_config(fused: bool, args):
         sinkhorn_regularization=args.reg, sinkhorn_iterations=args.sinkhorn_iterations,
         sinkhorn_threshold=args.threshold, sinkhorn_eval_freq=10, mu=args.mu, iter_count=args.iter_count,
         frank_wolfe_iter_count=args.fw_iter_count, use_sparse_adjacency=True,
-        hungarian_method=HungarianMethod.DOUBLE_GREEDY, sinkhorn_fused=fused,
+        hungarian_method=HungarianMethod[lap], sinkhorn_fused=fused, gradient_precision=precision,
     )
 
 
@@ -109,6 +118,44 @@ def measured_traffic(n: int):
     if data.get("n") != n or entry is None:
         return None
     return entry["read_bytes"] + entry["write_bytes"]
+
+
+def gradient_error_sample(A, B, P, rows: int = 16, seed: int = 0):
+    """The precision-form gradient -2 A P B (no distance or regulariser
+    term) on the device against FP64 on `rows` sampled rows, and against the
+    FP32 CSR gradient on the whole matrix.  FP64 row i is
+    -2 (sum_{k in N_A(i)} P[k, :]) B, from the rows of P it needs."""
+    import numpy as np
+    import scipy.sparse
+    import torch
+
+    from cugal_amd.pred import _gradient_into
+
+    n = P.shape[0]
+    g16 = torch.empty_like(P)
+    g32 = torch.empty_like(P)
+    _gradient_into(g16, A, B, P, None, 0, 1.0, precision="bf16x2")
+    _gradient_into(g32, A, B, P, None, 0, 1.0, precision="fp32")
+    scale = float(g32.abs().max())
+    vs_fp32 = float((g16 - g32).abs().max()) / scale
+    g32 = None
+    brp = B.row_pointers.cpu().numpy()
+    bcol = B.col_indices.cpu().numpy()
+    Bs = scipy.sparse.csr_matrix((np.ones(len(bcol)), bcol, brp), shape=(n, n))
+    arp = A.row_pointers.cpu().numpy()
+    acol = A.col_indices.cpu().numpy()
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for i in rng.choice(n, size=min(rows, n), replace=False):
+        nbrs = acol[arp[i]:arp[i + 1]]
+        if len(nbrs) == 0:
+            continue
+        y = P[torch.as_tensor(nbrs, device=P.device, dtype=torch.long)].double().sum(0).cpu().numpy()
+        ref = -2.0 * (Bs.T @ y)  # B symmetric: y B = B^T y
+        got = g16[int(i)].double().cpu().numpy()
+        worst = max(worst, float(np.max(np.abs(got - ref))) / max(float(np.max(np.abs(ref))), 1e-300))
+    return {"max_err_vs_fp64_sampled_rows": worst, "rows": int(min(rows, n)),
+            "max_err_vs_fp32_csr": vs_fp32, "norm": "max |error| / max |reference| of the row or matrix"}
 
 
 def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: int):
@@ -198,7 +245,9 @@ def main() -> int:
     ap.add_argument("--sinkhorn-iterations", type=int, default=500)
     ap.add_argument("--iter-count", type=int, default=15)
     ap.add_argument("--fw-iter-count", type=int, default=10)
+    ap.add_argument("--workload", choices=("ws32768", "ba8192", "sbm65536"), default="ws32768")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain two-sweep comparison run")
+    ap.add_argument("--no-precision-form", action="store_true", help="skip the matrix-core gradient run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -215,6 +264,10 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a ROCm device"
     if args.probe:
         return probe(args)
+    if args.workload == "ba8192":
+        return run_ba8192(args)
+    if args.workload == "sbm65536":
+        return run_sbm65536(args)
     log(f"device {torch.cuda.get_device_name(0)}; generating WS(n={args.n}, k={args.k}, p={args.p})")
     t = time.perf_counter()
     inst = watts_strogatz(args.n, args.k, args.p, args.noise, seed=args.seed)
@@ -255,6 +308,26 @@ def main() -> int:
                  "sweep_ms": pms, "sweep_launches": pcnt}
         log(f"plain two-sweep: {e:.2f}s acc {a:.4f}")
 
+    precision = None
+    if not args.no_precision_form:
+        e, a, xprof, _, Px = run_alignment(inst, make_config(True, args, precision="bf16x2"), False, inputs)
+        from cugal_amd.profile import Phase
+
+        grad_s = xprof.phase_times.get(Phase.GRADIENT, 0.0)
+        steps = xprof.frank_wolfe_iterations
+        npad = -(-args.n // 256) * 256
+        issued = steps * 8.0 * npad ** 3 / grad_s / 1e12 if grad_s else None
+        precision = {
+            "seconds": e, "accuracy": a, "gradient_seconds": grad_s, "frank_wolfe_steps": steps,
+            "mfma_tflops_issued": issued,
+            "mfma_tflops_algorithmic": steps * 4.0 * args.n ** 3 / grad_s / 1e12 if grad_s else None,
+            "mfma_peak_tflops": 2500.0, "mfma_frac": (issued / 2500.0) if issued else None,
+            "gradient_error": gradient_error_sample(inputs[0], inputs[1], Px),
+        }
+        Px = None
+        log(f"precision form (MFMA gradient): {e:.2f}s acc {a:.4f}, gradient {grad_s:.2f}s "
+            f"({issued:.0f} TFLOP/s issued)")
+
     n = args.n
     avg_sweep_ms = sweep_ms / max(1, sweep_n)
     sweep_bytes = 4.0 * n * n
@@ -288,10 +361,129 @@ def main() -> int:
         "accuracy": sum(accs) / len(accs), "sinkhorn_iterations": iters_total,
         "sinkhorn_gbs": achieved, "phase_times": phase, "api_extra_s": api_extra, "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
+        "precision_form": precision,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _contract_line(value, args, workload: str, n: int, extra: dict) -> str:
+    out = {
+        "metric": "align time (s) n=32k mixed vs FP32 plain @ equal node acc; Sinkhorn HBM GB/s",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": args.warmup, "ms_per_step": value * 1e3,
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": workload, "n": n, "parallelism": "single GPU"},
+    }
+    out.update(extra)
+    return json.dumps(out)
+
+
+def run_ba8192(args) -> int:
+    """Barabasi-Albert n=8192, m=5, 2% edge noise; the alignment with the
+    exact device Hungarian (DENSE), the same P rounded by scipy's
+    linear_sum_assignment on the host and by the greedy walk."""
+    import numpy as np
+    import scipy.optimize
+    import torch
+
+    from cugal_amd.config import HungarianMethod
+    from cugal_amd.greedy_lap import greedy_lap_device
+    from cugal_amd.hungarian import dense_hungarian
+    from cugal_amd.synthetic import alignment_accuracy, barabasi_albert
+
+    n = 8192
+    inst = barabasi_albert(n, 5, 0.02, seed=args.seed)
+    small = barabasi_albert(512, 5, 0.02, seed=1)
+    warm = argparse.Namespace(**{**vars(args), "iter_count": 1, "fw_iter_count": 2})
+    run_alignment(small, make_config(True, warm, lap="DENSE"), False)
+    inputs = device_inputs(inst, "cuda:0")
+    e, acc, prof, (ms, cnt), P = run_alignment(inst, make_config(True, args, lap="DENSE"), True, inputs)
+    iters = sum(p.iteration_count for p in prof.sinkhorn_profiles)
+    col_dense = dense_hungarian(P, maximize=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    col_dense, (free_rows, dsteps) = dense_hungarian(P, maximize=True, stats=True)
+    torch.cuda.synchronize()
+    t_dense = time.perf_counter() - t0
+    Ph = P.cpu().numpy().astype(np.float64)
+    t0 = time.perf_counter()
+    _, col_scipy = scipy.optimize.linear_sum_assignment(Ph, maximize=True)
+    t_scipy = time.perf_counter() - t0
+    Q = P.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    col_greedy = greedy_lap_device(Q, HungarianMethod.DOUBLE_GREEDY)
+    torch.cuda.synchronize()
+    t_greedy = time.perf_counter() - t0
+    col_dense = col_dense.cpu().numpy()
+    col_greedy = col_greedy.cpu().numpy()
+    avg = ms / max(1, cnt)
+    extra = {
+        "accuracy": acc, "sinkhorn_iterations": iters,
+        "phase_times": {k.value: v for k, v in prof.phase_times.items()},
+        "lap": {
+            "hungarian_device_s": t_dense, "hungarian_free_rows_after_warm_start": free_rows,
+            "hungarian_dijkstra_steps": dsteps, "scipy_s": t_scipy,
+            "hungarian_equals_scipy": bool(np.array_equal(col_dense, col_scipy)),
+            "objective_hungarian": float(Ph[np.arange(n), col_dense].sum()),
+            "objective_scipy": float(Ph[np.arange(n), col_scipy].sum()),
+            "greedy_s": t_greedy, "accuracy_greedy": alignment_accuracy(inst.source_mapping, col_greedy),
+            "accuracy_hungarian": alignment_accuracy(inst.source_mapping, col_dense),
+        },
+        "sinkhorn_gbs": 4.0 * n * n / (avg * 1e-3) / 1e9 if cnt else None,
+    }
+    log(f"ba8192: {e:.2f}s acc {acc:.4f}; hungarian {t_dense * 1e3:.1f} ms vs scipy {t_scipy * 1e3:.0f} ms, "
+        f"equal={extra['lap']['hungarian_equals_scipy']}")
+    print(_contract_line(e, args, f"barabasi_albert n={n} m=5, 2% edge noise, seeded permutation; LOG reg "
+                                  f"{args.reg}, {args.iter_count}x{args.fw_iter_count} FW, DENSE (device Hungarian)",
+                         n, extra), flush=True)
+    return 0
+
+
+def run_sbm65536(args) -> int:
+    """Stochastic block model: 64 blocks of 1024 nodes, intra/inter degrees
+    48/16, 1% edge noise; 15 lambda stages of one Frank-Wolfe step, greedy
+    LAP.  P (17 GB) and the cost stay in HBM; the Sinkhorn uses the
+    two-sweep kernels (the fused sweep keeps v in LDS up to n = 32768)."""
+    import torch
+
+    from cugal_amd.profile import Phase
+    from cugal_amd.synthetic import stochastic_block
+
+    t = time.perf_counter()
+    inst = stochastic_block(64, 1024, 48.0, 16.0, 0.01, seed=args.seed)
+    n = inst.source.number_of_nodes()
+    log(f"sbm instance n={n} in {time.perf_counter() - t:.1f}s ({inst.source.number_of_edges()} edges)")
+    cfg_args = argparse.Namespace(**{**vars(args), "iter_count": 15, "fw_iter_count": 1})
+    small = watts_small()
+    run_alignment(small, make_config(True, argparse.Namespace(**{**vars(args), "iter_count": 1,
+                                                                 "fw_iter_count": 2})), False)
+    inputs = device_inputs(inst, "cuda:0")
+    e, acc, prof, (ms, cnt), P = run_alignment(inst, make_config(True, cfg_args), True, inputs)
+    P = None
+    avg = ms / max(1, cnt)
+    gbs = 4.0 * n * n / (avg * 1e-3) / 1e9 if cnt else None
+    extra = {
+        "accuracy": acc, "sinkhorn_iterations": sum(p.iteration_count for p in prof.sinkhorn_profiles),
+        "phase_times": {k.value: v for k, v in prof.phase_times.items()},
+        "sinkhorn_gbs": gbs, "sinkhorn_sweep_avg_ms": avg, "sinkhorn_sweeps": cnt,
+        "roofline": {"kernel": "sk_col_sweep / sk_row_sweep (two reads of C per iteration)", "bound": "hbm",
+                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS if gbs else None, "traffic": None},
+    }
+    log(f"sbm65536: {e:.2f}s acc {acc:.4f}, sinkhorn sweeps {gbs:.0f} GB/s, "
+        f"phases {[(k.value, round(v, 2)) for k, v in prof.phase_times.items()]}")
+    print(_contract_line(e, args, f"stochastic_block 64 x 1024, degrees 48/16, 1% edge noise, seeded "
+                                  f"permutation; LOG reg {args.reg}, 15 lambda stages x 1 FW step, DOUBLE_GREEDY",
+                         n, extra), flush=True)
+    return 0
+
+
+def watts_small():
+    from cugal_amd.synthetic import watts_strogatz
+
+    return watts_strogatz(512, 10, 0.1, 0.01, seed=1)
 
 
 if __name__ == "__main__":
