@@ -197,8 +197,9 @@ def probe(args) -> int:
     from cugal_amd.config import HungarianMethod
     from cugal_amd.feature_extraction import Features, node_features
     from cugal_amd.greedy_lap import greedy_lap_device
-    from cugal_amd.pred import _gradient_into, update_quasi_permutation_log
-    from cugal_amd.sinkhorn import LogKernel, solve_log_native
+    from cugal_amd.hungarian import dense_hungarian
+    from cugal_amd.pred import _gradient_into, update_quasi_permutation_log, update_quasi_permutation_standard
+    from cugal_amd.sinkhorn import LogKernel, StandardKernel, solve_log_native, solve_standard_native
     from cugal_amd.synthetic import watts_strogatz
 
     n = args.n
@@ -212,13 +213,33 @@ def probe(args) -> int:
     P /= P.sum(1, keepdim=True)
     G = torch.empty_like(P)
     for lam in (3, 4):
+        _gradient_into(G, A, B, P, feats, lam, 1.0, precision="bf16x2")
+    for lam in (3, 4):
         _gradient_into(G, A, B, P, feats, lam, 1.0)
+    us = torch.full((n,), 1.0 / n, device=dev)
+    vs = torch.full((n,), 1.0 / n, device=dev)
+    solve_standard_native(G, 4.0 * args.reg, us, vs, 10, 0.0, 10, None, fused=True)
     u = torch.zeros(n, device=dev)
     v = torch.zeros(n, device=dev)
     solve_log_native(G, args.reg, u, v, 20, 0.0, 10, None, fused=True)
     solve_log_native(G, args.reg, u, v, 10, 0.0, 10, None, fused=False)
     gap = torch.empty(1, dtype=torch.float64, device=dev)
+    Q = P.clone()
+    update_quasi_permutation_standard(Q, StandardKernel(G, 4.0 * args.reg), us, vs, 0.5, gap=gap)
     update_quasi_permutation_log(P, LogKernel(G, args.reg), u, v, 0.5, gap=gap)
+    Q.copy_(P)
+    greedy_lap_device(Q, HungarianMethod.PARALLEL_GREEDY)
+    Q.copy_(P)
+    greedy_lap_device(Q, HungarianMethod.MORE_RAND)
+    # The exact LAP on a quasi-permutation with 5% rival entries (the shape
+    # the alignment produces; a uniform random matrix would take far longer).
+    perm = torch.randperm(n, generator=g, device=dev)
+    Q = torch.rand((n, n), generator=g, device=dev) * (0.2 / n)
+    Q[torch.arange(n, device=dev), perm] += 0.5
+    swap = torch.randperm(n, generator=g, device=dev)[: n // 20]
+    Q[swap, perm[torch.roll(swap, 1)]] += 0.6
+    dense_hungarian(Q, maximize=True)
+    Q = None
     greedy_lap_device(P, HungarianMethod.DOUBLE_GREEDY)
     copy = torch.empty_like(P)
     for _ in range(3):
