@@ -551,15 +551,14 @@ LapWs lap_carve(char* base, int64_t n, size_t bytes, size_t* used = nullptr) {
     return w;
 }
 
-// Raise the dynamic LDS limit of a walk kernel once, when its n-bit taken
-// bitmap needs more than the default.
-void allow_lds(const void* kernel, size_t lds, bool& done) {
-    if (done || lds <= 32768) return;
-    int dev = 0, v = 65536;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, v);
-    done = true;
+// Raise the dynamic LDS limit of a walk kernel to what its n-bit taken
+// bitmap needs (the kernels also hold a little static LDS, so exactly the
+// dynamic bytes are requested; the attribute only grows).
+int allow_lds(const void* kernel, size_t lds, size_t& granted) {
+    if (lds <= 32768 || lds <= granted) return CG_OK;
+    CG_CHECK_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    granted = lds;
+    return CG_OK;
 }
 
 enum class Walk { kGreedy, kDouble, kParallel, kRand, kMoreRand };
@@ -603,8 +602,8 @@ int lap_run(float* P, int64_t n, Walk method, const int32_t* order_in, const dou
     if (int rc = check_launch("fill_int")) return rc;
     const size_t lds = sizeof(uint32_t) * static_cast<size_t>(div_up(n, 32));
     if (method == Walk::kMoreRand) {
-        static bool sample_attr = false;
-        allow_lds(reinterpret_cast<const void*>(&sample_walk), lds, sample_attr);
+        static size_t sample_attr = 0;
+        if (int rc = allow_lds(reinterpret_cast<const void*>(&sample_walk), lds, sample_attr)) return rc;
         hipLaunchKernelGGL(sample_walk, dim3(1), dim3(kSampleThreads), lds, s, P, ni, order, uniforms, assignment,
                            w.step_col, w.step_row);
         if (int rc = check_launch("sample_walk")) return rc;
@@ -619,8 +618,8 @@ int lap_run(float* P, int64_t n, Walk method, const int32_t* order_in, const dou
             walk_count = w.walk_count;
             taken_init = w.taken_init;
         }
-        static bool walk_attr = false;
-        allow_lds(reinterpret_cast<const void*>(&greedy_walk), lds, walk_attr);
+        static size_t walk_attr = 0;
+        if (int rc = allow_lds(reinterpret_cast<const void*>(&greedy_walk), lds, walk_attr)) return rc;
         hipLaunchKernelGGL(greedy_walk, dim3(1), dim3(kWalkThreads), lds, s, P, ni, order, w.cval, w.cidx, assignment,
                            w.step_col, w.step_row, w.nfall, walk_count, taken_init);
         if (int rc = check_launch("greedy_walk")) return rc;

@@ -95,16 +95,17 @@ __global__ void sk_finish(const float* __restrict__ u2, const float* __restrict_
 // n^2, still far inside FP32.)  The next row is loaded while the current one
 // is reduced (two register buffers).
 // ---------------------------------------------------------------------------
-template <int T, int G>
+// FULL: n == 4 G T, every column group exists (no bounds tests).
+template <int T, int G, bool FULL = false>
 __device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __restrict__ row, int n) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + threadIdx.x) * 4;
-        r[g] = col < n ? ld_stream4(row + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+        r[g] = (FULL || col < n) ? ld_stream4(row + col) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool FULL = false>
 __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], int i, int n, float scale2,
                                           const float* __restrict__ sv, float2* __restrict__ red,
                                           float* __restrict__ u2, float mom) {
@@ -115,7 +116,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        if (col < n) {
+        if (FULL || col < n) {
             const float4 vv = *reinterpret_cast<const float4*>(sv + col);
             r[g].x = fmaf(r[g].x, scale2, vv.x);
             r[g].y = fmaf(r[g].y, scale2, vv.y);
@@ -127,7 +128,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
     float ls = 0.0f;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if ((g * T + tid) * 4 < n)
+        if (FULL || (g * T + tid) * 4 < n)
             ls += (exp2_fast(r[g].x - lm) + exp2_fast(r[g].y - lm)) + (exp2_fast(r[g].z - lm) + exp2_fast(r[g].w - lm));
     }
     const Lse w = wave_lse({lm, ls});
@@ -145,7 +146,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
     if (tid == 0) u2[i] = ui;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if ((g * T + tid) * 4 < n) {
+        if (FULL || (g * T + tid) * 4 < n) {
             cs[4 * g + 0] += exp2_fast(r[g].x + ui);
             cs[4 * g + 1] += exp2_fast(r[g].y + ui);
             cs[4 * g + 2] += exp2_fast(r[g].z + ui);
@@ -154,17 +155,75 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
     }
 }
 
-template <int T, int G>
+// The same row step with the previous u2_i as the shift instead of the row
+// maximum: the sweep follows a v-update, so every e_j = 2^(K2_ij + v2_j +
+// u2_old_i) is a plan entry <= 1 and their sum S (the row mass) is at most
+// n.  Then u2_i = u2_old - m log2 S, and the column terms 2^(K2 + v2 + u2_i)
+// are e_j S^-m: one exponential per element instead of two, and no max
+// pass.  If S leaves [2^-100, 2^100] (a row whose mass collapsed or
+// exploded since the last solve; the decision is block-uniform) the row
+// contributes nothing here and is queued; the block redoes it with the
+// max-shifted step after its other rows.  Returns false for a queued row.
+template <int T, int G, bool FULL>
+__device__ __forceinline__ bool fused_row_shift(float4 (&r)[G], float (&cs)[4 * G], int i, int n, float scale2,
+                                                const float* __restrict__ sv, float2* __restrict__ red,
+                                                float* __restrict__ u2, float mom) {
+    constexpr int W = T / kWave;
+    const int tid = threadIdx.x;
+    const float u_old = u2[i];
+    float ls = 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (FULL || col < n) {
+            const float4 vv = *reinterpret_cast<const float4*>(sv + col);
+            r[g].x = exp2_fast(fmaf(r[g].x, scale2, vv.x) + u_old);
+            r[g].y = exp2_fast(fmaf(r[g].y, scale2, vv.y) + u_old);
+            r[g].z = exp2_fast(fmaf(r[g].z, scale2, vv.z) + u_old);
+            r[g].w = exp2_fast(fmaf(r[g].w, scale2, vv.w) + u_old);
+            ls += (r[g].x + r[g].y) + (r[g].z + r[g].w);
+        }
+    }
+    ls = wave_sum(ls);
+    float2* slot = red + (i & 1) * W;
+    if ((tid & (kWave - 1)) == 0) slot[tid / kWave] = make_float2(0.0f, ls);
+    __syncthreads();
+    float S = 0.0f;
+#pragma unroll
+    for (int k = 0; k < W; ++k) S += slot[k].y;
+    if (!(S >= 0x1p-100f && S <= 0x1p100f)) return false;
+    const float l2 = log2_fast(S);
+    if (tid == 0) u2[i] = u_old - mom * l2;
+    const float f = (mom == 1.0f) ? 1.0f / S : exp2_fast(-mom * l2);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (FULL || (g * T + tid) * 4 < n) {
+            cs[4 * g + 0] = fmaf(r[g].x, f, cs[4 * g + 0]);
+            cs[4 * g + 1] = fmaf(r[g].y, f, cs[4 * g + 1]);
+            cs[4 * g + 2] = fmaf(r[g].z, f, cs[4 * g + 2]);
+            cs[4 * g + 3] = fmaf(r[g].w, f, cs[4 * g + 3]);
+        }
+    }
+    return true;
+}
+
+constexpr int kFusedMaxRowsPerBlock = 128;  // n <= 32768 over 256 blocks
+
+
+template <int T, int G, bool FULL>
 __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C, int n, float scale2,
                                                    const float* __restrict__ v2, float* __restrict__ u2,
                                                    float* __restrict__ part, int rows_per_blk,
                                                    const SinkCtl* __restrict__ ctl) {
     extern __shared__ float smem[];
+    __shared__ int queued[kFusedMaxRowsPerBlock];
+    __shared__ int nqueued;
     if (ctl->done) return;
     const float mom = ctl->momentum;
     const int tid = threadIdx.x;
     float* sv = smem;
     float2* red = reinterpret_cast<float2*>(smem + round_up(n, 4));
+    if (tid == 0) nqueued = 0;
     for (int j = tid * 4; j < n; j += T * 4)
         *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v2 + j);
     __syncthreads();
@@ -179,13 +238,14 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
     if constexpr (4 * G * T <= 24 * 1024) {
         // Two register buffers: row i+1 is in flight while row i is reduced.
         float4 a[G], b[G];
-        if (r0 < r1) fused_load_row<T, G>(a, C + r0 * ld, n);
+        if (r0 < r1) fused_load_row<T, G, FULL>(a, C + r0 * ld, n);
         for (int i = r0; i < r1; i += 2) {
-            if (i + 1 < r1) fused_load_row<T, G>(b, C + (i + 1) * ld, n);
-            fused_row<T, G>(a, cs, i, n, scale2, sv, red, u2, mom);
+            if (i + 1 < r1) fused_load_row<T, G, FULL>(b, C + (i + 1) * ld, n);
+            if (!fused_row_shift<T, G, FULL>(a, cs, i, n, scale2, sv, red, u2, mom) && tid == 0) queued[nqueued++] = i;
             if (i + 1 >= r1) break;
-            if (i + 2 < r1) fused_load_row<T, G>(a, C + (i + 2) * ld, n);
-            fused_row<T, G>(b, cs, i + 1, n, scale2, sv, red, u2, mom);
+            if (i + 2 < r1) fused_load_row<T, G, FULL>(a, C + (i + 2) * ld, n);
+            if (!fused_row_shift<T, G, FULL>(b, cs, i + 1, n, scale2, sv, red, u2, mom) && tid == 0)
+                queued[nqueued++] = i + 1;
         }
     } else {
         // 32 columns per thread: a second row buffer would not fit in the 128
@@ -193,9 +253,20 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
         // waves' outstanding loads instead.
         float4 a[G];
         for (int i = r0; i < r1; ++i) {
-            fused_load_row<T, G>(a, C + i * ld, n);
-            fused_row<T, G>(a, cs, i, n, scale2, sv, red, u2, mom);
+            fused_load_row<T, G, FULL>(a, C + i * ld, n);
+            if (!fused_row_shift<T, G, FULL>(a, cs, i, n, scale2, sv, red, u2, mom) && tid == 0) queued[nqueued++] = i;
         }
+    }
+
+    // Rows whose mass left the shifted range, with the exact max shift.
+    __syncthreads();
+    const int nq = nqueued;
+    for (int q = 0; q < nq; ++q) {
+        const int i = queued[q];
+        float4 a[G];
+        fused_load_row<T, G>(a, C + i * ld, n);
+        fused_row<T, G>(a, cs, i, n, scale2, sv, red, u2, mom);
+        __syncthreads();  // queued rows may share a reduction slot
     }
 
     float* out = part + static_cast<int64_t>(blockIdx.x) * n;
@@ -393,7 +464,9 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         const float2* part = static_cast<const float2*>(part_raw);
         m = kNegBig;
         if (j < n) {
+#pragma unroll 8
             for (int b = wave; b < nblk; b += kCombWaves) m = fmaxf(m, part[static_cast<int64_t>(b) * n + j].x);
+#pragma unroll 8
             for (int b = wave; b < nblk; b += kCombWaves) {
                 const float2 p = part[static_cast<int64_t>(b) * n + j];
                 sum += static_cast<double>(p.y) * exp2(static_cast<double>(p.x) - static_cast<double>(m));
@@ -401,8 +474,10 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         }
     } else {
         const float* part = static_cast<const float*>(part_raw);
-        if (j < n)
+        if (j < n) {
+#pragma unroll 8
             for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
+        }
     }
     rm[wave][lane] = m;
     rs[wave][lane] = sum;
@@ -614,8 +689,10 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
     const int wave = threadIdx.x / kCombCols;
     const int j = blockIdx.x * kCombCols + lane;
     double sum = 0.0;
-    if (j < n)
+    if (j < n) {
+#pragma unroll 8
         for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
+    }
     rs[wave][lane] = sum;
     __syncthreads();
     double err = 0.0;
@@ -770,17 +847,25 @@ template <int T, int G>
 int launch_fused_tg(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
                     const SinkCtl* ctl, hipStream_t s) {
     const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float2) * 2 * (T / kWave);
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
-        attr_set = true;
-    }
-    if (lds > static_cast<size_t>(max_lds_bytes()))
+    // The kernel also holds a small static LDS queue: ask for exactly the
+    // dynamic bytes this launch needs (the attribute only grows).
+    static size_t attr_bytes = 0;
+    if (lds + 1024 > static_cast<size_t>(max_lds_bytes()))
         return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
                     max_lds_bytes());
-    hipLaunchKernelGGL((sk_fused_sweep<T, G>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
-                       p.rows_per_blk, ctl);
+    if (lds > attr_bytes) {
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G, true>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        attr_bytes = lds;
+    }
+    if (n == 4 * G * T)
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, true>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, ctl);
+    else
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, false>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, ctl);
     return check_launch("sk_fused_sweep");
 }
 
@@ -828,15 +913,15 @@ template <int T, int G>
 int launch_std_fused_tg(const float* C, int n, float scale2, const float* v, float* u, float* part, const Plan& p,
                         const SinkCtl* ctl, hipStream_t s) {
     const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float) * 2 * (T / kWave);
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
-        attr_set = true;
-    }
+    static size_t attr_bytes = 0;
     if (lds > static_cast<size_t>(max_lds_bytes()))
         return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
                     max_lds_bytes());
+    if (lds > attr_bytes) {
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        attr_bytes = lds;
+    }
     hipLaunchKernelGGL((std_fused_sweep<T, G>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v, u, part,
                        p.rows_per_blk, ctl);
     return check_launch("std_fused_sweep");

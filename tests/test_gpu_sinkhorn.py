@@ -86,6 +86,27 @@ def test_fused_equals_two_sweep(device):
     np.testing.assert_allclose(a[1].numpy(), b[1].numpy(), atol=2e-5)
 
 
+def test_fused_rows_out_of_shifted_range(device):
+    # Warm start from one cost, then rows 5 and 900 shift by +-200 (C/reg =
+    # -+667): their mass under the previous duals under/overflows FP32, so
+    # the fused sweep redoes them with the exact max shift.  Same iterates
+    # as the two-sweep path and as the FP64 oracle.
+    n, reg = 1024, 0.3
+    C1 = _cost(n, seed=21, scale=3.0)
+    u0, v0, _, _ = _run_gpu(C1, reg, 40, 0.0, 10, True, device)
+    C2 = C1.clone()
+    C2[5] += 200.0
+    C2[900] -= 200.0
+    a = _run_gpu(C2, reg, 30, 0.0, 10, True, device, u0=u0, v0=v0)
+    b = _run_gpu(C2, reg, 30, 0.0, 10, False, device, u0=u0, v0=v0)
+    _, ru, rv, _ = _run_oracle(C2, reg, 30, 0.0, 10, u0=u0, v0=v0)
+    assert np.isfinite(a[0].numpy()).all() and np.isfinite(a[1].numpy()).all()
+    np.testing.assert_allclose(a[0].numpy(), b[0].numpy(), atol=1e-3)
+    np.testing.assert_allclose(a[1].numpy(), b[1].numpy(), atol=1e-3)
+    np.testing.assert_allclose(a[0].numpy(), ru.numpy(), atol=2e-3)
+    np.testing.assert_allclose(a[1].numpy(), rv.numpy(), atol=2e-3)
+
+
 def test_threshold_stop_and_warm_start(device):
     n = 512
     C1 = _cost(n, seed=11, scale=2.0)
