@@ -45,8 +45,8 @@ _SIGNATURES = {
     "cg_regularize": (c_int, [c_p, c_p, c_i64, c_int, c_p]),
     "cg_gradient_workspace": (c_sz, [c_i64]),
     "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
-    "cg_dense_adjacency_elements": (c_sz, [c_i64]),
-    "cg_adjacency_densify_bf16": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "cg_adjacency_bits_words": (c_sz, [c_i64]),
+    "cg_adjacency_pack_bits": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_gradient_mfma_workspace": (c_sz, [c_i64]),
     "cg_gradient_mfma": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
     "cg_sinkhorn_log_step": (c_int, [c_p, c_i64, c_p, c_p, c_f, c_p]),

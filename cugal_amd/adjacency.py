@@ -53,7 +53,7 @@ class Adjacency:
     row_pointers: torch.Tensor
     """Offset in col_indices where each row begins; length n + 1 (int32)."""
 
-    _dense_bf16: torch.Tensor | None = field(default=None, init=False, repr=False, compare=False)
+    _packed_bits: torch.Tensor | None = field(default=None, init=False, repr=False, compare=False)
 
     @classmethod
     def from_dense(cls, dense: torch.Tensor) -> "Adjacency":
@@ -87,19 +87,20 @@ class Adjacency:
         dense[rows, self.col_indices.to(torch.int64)] = 1
         return dense
 
-    def dense_bf16(self) -> torch.Tensor:
-        """The 0/1 adjacency as a dense BF16 matrix (exact), zero padded to
-        n rounded up to 256 in both dimensions: the operand of the MFMA
-        gradient.  Built once on the device and cached on the object."""
-        if self._dense_bf16 is None:
+    def packed_bits(self) -> torch.Tensor:
+        """The 0/1 adjacency as bits, zero padded to np = n rounded up to
+        256: word (k // 32) * np + i holds bit k % 32 of row i (int32
+        storage of uint32 words).  The operand of the matrix-core gradient;
+        built once on the device and cached on the object."""
+        if self._packed_bits is None:
             n = self.number_of_nodes()
             dev = self.col_indices.device
-            elems = int(nat.query("cg_dense_adjacency_elements", n))
-            out = torch.empty(elems, dtype=torch.bfloat16, device=dev)
-            nat.call("cg_adjacency_densify_bf16", nat.ptr(self.col_indices), nat.ptr(self.row_pointers), n,
+            words = int(nat.query("cg_adjacency_bits_words", n))
+            out = torch.empty(words, dtype=torch.int32, device=dev)
+            nat.call("cg_adjacency_pack_bits", nat.ptr(self.col_indices), nat.ptr(self.row_pointers), n,
                      nat.ptr(out), nat.stream_of(dev))
-            self._dense_bf16 = out
-        return self._dense_bf16
+            self._packed_bits = out
+        return self._packed_bits
 
     def byte_size(self) -> int:
         return 4 * len(self.col_indices) + 4 * len(self.row_pointers)

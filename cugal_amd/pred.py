@@ -63,7 +63,7 @@ def _gradient_into(
     if precision == "bf16x2":
         ws = nat.workspace(dev, nat.query("cg_gradient_mfma_workspace", n), "gradient_mfma")
         nat.call(
-            "cg_gradient_mfma", nat.ptr(A.dense_bf16()), nat.ptr(B.dense_bf16()), n, nat.ptr(P), nat.ptr(fs),
+            "cg_gradient_mfma", nat.ptr(A.packed_bits()), nat.ptr(B.packed_bits()), n, nat.ptr(P), nat.ptr(fs),
             nat.ptr(ft), int(iteration), float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
         )
     elif precision == "fp32":

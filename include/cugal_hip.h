@@ -201,38 +201,22 @@ int cg_sinkhorn_unrolled_backward(const float* C, int64_t n, float regularizatio
 
 /* The precision form of cg_gradient (same quantity, cugal/pred.py:38-74):
  * G = -2 A P B + D + regularizer_factor * lambda_ * (1 - 2P) with A, B as
- * dense 0/1 BF16 matrices (exact) and P split into BF16 hi + lo, products
- * on the matrix cores (v_mfma_f32_32x32x16_bf16) accumulated in FP32.
- * Where the reference multiplies dense adjacencies in FP32 (torch.matmul,
- * cugal/pred.py:45-46) this reads half the bytes per operand and runs on
- * MFMA; each product carries a relative error of about 2^-16 from the hi+lo
- * split.  a_dense, b_dense: cg_dense_adjacency_elements(n) uint16 (BF16
- * bits) each, from cg_adjacency_densify_bf16: the adjacency zero padded to
- * np = n rounded up to 256 (np x np, row-major).  Undirected graphs only. */
-size_t cg_dense_adjacency_elements(int64_t n);
-int cg_adjacency_densify_bf16(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint16_t* out,
-                              cg_stream_t stream);
+ * exact 0/1 BF16 operands and P split into BF16 hi + lo, products on the
+ * matrix cores (v_mfma_f32_32x32x16_bf16) accumulated in FP32.  Where the
+ * reference multiplies dense adjacencies in FP32 (torch.matmul,
+ * cugal/pred.py:45-46) this runs on MFMA; each product carries a relative
+ * error of about 2^-16 from the hi+lo split.  a_bits, b_bits:
+ * cg_adjacency_bits_words(n) words each, from cg_adjacency_pack_bits: the
+ * adjacency as bits, zero padded to np = n rounded up to 256; word
+ * (k / 32) * np + i holds bits k % 32 of row i.  Undirected graphs only. */
+size_t cg_adjacency_bits_words(int64_t n);
+int cg_adjacency_pack_bits(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint32_t* out,
+                           cg_stream_t stream);
 size_t cg_gradient_mfma_workspace(int64_t n);
-int cg_gradient_mfma(const uint16_t* a_dense, const uint16_t* b_dense, int64_t n, const float* P,
+int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P,
                      const float* source_features, const float* target_features, int lambda_,
                      float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                      cg_stream_t stream);
-
-/* Replaces solve_standard (cugal/sinkhorn.py:122-144): with K = exp(-C/reg)
- * (recomputed from C where read, never stored) and u, v the scalings,
- *   v_j = v_j^(1-m) (1 / (sum_i u_i K_ij + 1e-16))^m,
- *   u_i = u_i^(1-m) (1 / (sum_j K_ij v_j + 1e-16))^m,
- * the column-marginal error sum_j |sum_i u_i K_ij v_j - 1| (FP64) at
- * iterations 0, eval_freq, 2 eval_freq, ..., and a stop below `threshold`;
- * Lehmann momentum as in cg_sinkhorn_log_solve.  u, v (device, n floats)
- * are read as the warm start and overwritten.  The fused path reads C once
- * per iteration; otherwise a row and a column sweep.  Workspace:
- * cg_sinkhorn_workspace(n, fused, max_iterations, eval_freq).  FP32 as the
- * reference's GPU run of this method (overflow/underflow of K as there). */
-int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, float* u, float* v,
-                               int max_iterations, double threshold, int eval_freq, int momentum_start, int fused,
-                               double* errors, int errors_capacity, int* error_count, int* iterations,
-                               void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Frank-Wolfe update.

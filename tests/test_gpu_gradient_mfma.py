@@ -36,14 +36,16 @@ def _case(device, n, p, seed):
     return A, B, P, Ad, Bd
 
 
-def test_dense_adjacency_is_exact(device):
-    n = 300  # padded to 512
+def test_packed_adjacency_is_exact(device):
+    n = 300  # padded to 512: word (k // 32) * 512 + i holds bit k % 32 of row i
     g = nx.erdos_renyi_graph(n, 0.05, seed=1)
     A = Adjacency.from_graph(g, device)
-    d = A.dense_bf16().view(512, 512).float().cpu()
-    want = torch.zeros((512, 512))
-    want[:n, :n] = torch.tensor(nx.to_numpy_array(g, nodelist=range(n)), dtype=torch.float32)
-    assert torch.equal(d, want)
+    words = A.packed_bits().cpu().numpy().view(np.uint32).reshape(512 // 32, 512)
+    k = np.arange(512)
+    bits = (words[k // 32, :] >> (k % 32)[:, None].astype(np.uint32)) & 1  # [k, i]
+    want = np.zeros((512, 512), dtype=np.uint32)
+    want[:n, :n] = nx.to_numpy_array(g, nodelist=range(n)).astype(np.uint32)
+    assert np.array_equal(bits.T, want)
 
 
 @pytest.mark.parametrize("sparse", [True, False])
