@@ -134,32 +134,39 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
                 const int e = rp[r + 1];
                 if (VEC) {
                     if (cq < n) {
-                        // Four neighbours per step: their indices, then their
-                        // segments, are in flight together; the sum keeps the
-                        // neighbour order.
+                        // Eight neighbours per step: their indices, then
+                        // their segments, are in flight together; the sum
+                        // keeps the neighbour order.
                         int k = b;
+                        for (; k + 8 <= e; k += 8) {
+                            int64_t kk[8];
+                            float4 x[8];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) kk[q] = col[k + q];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) x[q] = *reinterpret_cast<const float4*>(X + kk[q] * ldx + cq);
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                acc.x += x[q].x;
+                                acc.y += x[q].y;
+                                acc.z += x[q].z;
+                                acc.w += x[q].w;
+                            }
+                        }
                         for (; k + 4 <= e; k += 4) {
-                            const int64_t k0 = col[k], k1 = col[k + 1], k2 = col[k + 2], k3 = col[k + 3];
-                            const float4 x0 = *reinterpret_cast<const float4*>(X + k0 * ldx + cq);
-                            const float4 x1 = *reinterpret_cast<const float4*>(X + k1 * ldx + cq);
-                            const float4 x2 = *reinterpret_cast<const float4*>(X + k2 * ldx + cq);
-                            const float4 x3 = *reinterpret_cast<const float4*>(X + k3 * ldx + cq);
-                            acc.x += x0.x;
-                            acc.y += x0.y;
-                            acc.z += x0.z;
-                            acc.w += x0.w;
-                            acc.x += x1.x;
-                            acc.y += x1.y;
-                            acc.z += x1.z;
-                            acc.w += x1.w;
-                            acc.x += x2.x;
-                            acc.y += x2.y;
-                            acc.z += x2.z;
-                            acc.w += x2.w;
-                            acc.x += x3.x;
-                            acc.y += x3.y;
-                            acc.z += x3.z;
-                            acc.w += x3.w;
+                            int64_t kk[4];
+                            float4 x[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) kk[q] = col[k + q];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const float4*>(X + kk[q] * ldx + cq);
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                acc.x += x[q].x;
+                                acc.y += x[q].y;
+                                acc.z += x[q].z;
+                                acc.w += x[q].w;
+                            }
                         }
                         for (; k < e; ++k) {
                             const float4 x =
