@@ -10,9 +10,12 @@ and the assignment in HBM (`cugal_amd.pred.align_device`).  The API-level
 extras of `cugal()` -- building the CSR from networkx graphs and copying P
 to the host -- are timed separately and reported as `api_extra_s`.
 
-Reported: `value` = seconds per alignment on the fused path ("mixed": the
-cost is read once per Sinkhorn iteration), beside the plain two-sweep path
-on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
+Reported: `value` = seconds per alignment on the "mixed" path -- the fused
+Sinkhorn (the cost read once per iteration; FP32 potentials, FP64 column
+sums, marginal errors and Frank-Wolfe gap) with the gradient in the form
+the adjacency density calls for (`gradient_precision="auto"`: the CSR
+gathers here, the matrix cores for dense graphs) -- beside the plain FP32
+path (two-sweep Sinkhorn, CSR gradient) on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
 4 n^2 per sweep over the HIP-event time of the sweep launches) and the FP64
 CPU oracle timed on a bounded sample of the same Sinkhorn work.  The
 precision form (gradient on the matrix cores: 0/1 BF16 adjacency times BF16
@@ -45,7 +48,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_config(fused: bool, args, precision: str = "fp32", lap: str = "DOUBLE_GREEDY"):
+def make_config(fused: bool, args, precision: str = "auto", lap: str = "DOUBLE_GREEDY"):
     import torch
 
     from cugal_amd import Config, HungarianMethod, SinkhornMethod
@@ -323,7 +326,7 @@ def main() -> int:
 
     plain = None
     if not args.no_plain:
-        e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args), True, inputs)
+        e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args, precision="fp32"), True, inputs)
         plain = {"seconds": e, "accuracy": a,
                  "sinkhorn_iterations": sum(p.iteration_count for p in pprof.sinkhorn_profiles),
                  "sweep_ms": pms, "sweep_launches": pcnt}

@@ -122,9 +122,10 @@ class Config:
 
     gradient_precision: str = "fp32"
     """Arithmetic of the A P B contraction: "fp32" (CSR adjacency times
-    dense FP32 P) or "bf16x2" (dense 0/1 BF16 adjacency times P split into
-    BF16 hi + lo on the matrix cores, FP32 accumulation; pays off for dense
-    adjacencies)."""
+    dense FP32 P), "bf16x2" (the 0/1 adjacency, exact in BF16, times P split
+    into BF16 hi + lo on the matrix cores, FP32 accumulation) or "auto" (the
+    matrix cores when the adjacencies hold more than ~0.77 % of n^2
+    non-zeros, where they become the faster form, else "fp32")."""
 
     def convert_tensor(self, input: torch.Tensor) -> torch.Tensor:
         """Convert tensor to correct type and dtype."""

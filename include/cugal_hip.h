@@ -164,6 +164,22 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
                           int errors_capacity, int* error_count, int* iterations, float* sweep_ms,
                           int* sweep_launches, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
+/* Replaces solve_standard (cugal/sinkhorn.py:122-144): with K = exp(-C/reg)
+ * (recomputed from C where read, never stored) and u, v the scalings,
+ *   v_j = v_j^(1-m) (1 / (sum_i u_i K_ij + 1e-16))^m,
+ *   u_i = u_i^(1-m) (1 / (sum_j K_ij v_j + 1e-16))^m,
+ * the column-marginal error sum_j |sum_i u_i K_ij v_j - 1| (FP64) at
+ * iterations 0, eval_freq, 2 eval_freq, ..., and a stop below `threshold`;
+ * Lehmann momentum as in cg_sinkhorn_log_solve.  u, v (device, n floats)
+ * are read as the warm start and overwritten.  The fused path reads C once
+ * per iteration; otherwise a row and a column sweep.  Workspace:
+ * cg_sinkhorn_workspace(n, fused, max_iterations, eval_freq).  FP32 as the
+ * reference's GPU run of this method (overflow/underflow of K as there). */
+int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, float* u, float* v,
+                               int max_iterations, double threshold, int eval_freq, int momentum_start, int fused,
+                               double* errors, int errors_capacity, int* error_count, int* iterations,
+                               void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Differentiable unrolled log-Sinkhorn.  No reference counterpart: the
  * reference's only differentiable form is torch.logsumexp in its CPU branch
