@@ -117,7 +117,8 @@ def measured_traffic(n: int):
         data = json.load(open(path))
     except (OSError, ValueError):
         return None
-    entry = data.get("kernels", {}).get("sk_fused_sweep<1024, 8>")
+    kernels = data.get("kernels", {})
+    entry = kernels.get("sk_fused_sweep<1024, 8, true>") or kernels.get("sk_fused_sweep<1024, 8>")
     if data.get("n") != n or entry is None:
         return None
     return entry["read_bytes"] + entry["write_bytes"]
