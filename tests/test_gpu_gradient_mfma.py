@@ -117,3 +117,21 @@ def test_mfma_gradient_rate(device):
     print(f"\nmfma gradient n={n}: {dt * 1e3:.2f} ms, {4 * n**3 / dt / 1e12:.0f} TFLOP/s algorithmic "
           f"({8 * n**3 / dt / 1e12:.0f} TFLOP/s issued with the hi + lo split)")
     assert torch.isfinite(out).all()
+
+
+@pytest.mark.parametrize("p,want", [(0.002, "fp32"), (0.05, "bf16x2")])
+def test_auto_precision_runs_the_chosen_kernel(device, p, want):
+    # "auto" gives bit for bit the output of the kernel it selects.
+    from cugal_amd.pred import gradient_precision_for
+
+    n = 2048
+    A, B, P, _, _ = _case(device, n, p, seed=9)
+    assert gradient_precision_for(A, B, "auto") == want
+    Pd = P.to(device)
+    cfg = Config(device="cuda", dtype=torch.float32, mu=0.5)
+    feats = Features(node_features(A, cfg.mu), node_features(B, cfg.mu))
+    got = torch.empty((n, n), device=device)
+    ref = torch.empty((n, n), device=device)
+    _gradient_into(got, A, B, Pd, feats, 2, 1.0, precision="auto")
+    _gradient_into(ref, A, B, Pd, feats, 2, 1.0, precision=want)
+    assert torch.equal(got, ref)

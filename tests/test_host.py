@@ -131,3 +131,27 @@ def test_oracle_not_imported_by_library():
                         mod = getattr(node, "module", None) or ""
                         names = [a.name for a in node.names]
                         assert not mod.startswith("oracle") and not any(n.startswith("oracle") for n in names), f
+
+
+def test_auto_gradient_precision_follows_density():
+    # "auto" resolves on the host from the CSR sizes: the matrix cores above
+    # MFMA_DENSITY of the n^2 entries, the CSR gathers below; explicit
+    # choices pass through.
+    from cugal_amd.adjacency import Adjacency
+    from cugal_amd.pred import MFMA_DENSITY, gradient_precision_for
+
+    n = 1000
+
+    def adj(per_row):
+        col = torch.arange(n * per_row, dtype=torch.int32) % n
+        rp = torch.arange(0, n * per_row + 1, per_row, dtype=torch.int32)
+        return Adjacency(col, rp)
+
+    threshold = MFMA_DENSITY * n  # neighbours per row at the switch
+    sparse, dense = adj(int(threshold) - 1), adj(int(threshold) + 2)
+    assert gradient_precision_for(sparse, sparse, "auto") == "fp32"
+    assert gradient_precision_for(dense, dense, "auto") == "bf16x2"
+    # the mean of the two graphs' sizes decides
+    assert gradient_precision_for(adj(1), adj(2 * int(threshold) + 4), "auto") == "bf16x2"
+    for explicit in ("fp32", "bf16x2"):
+        assert gradient_precision_for(dense, sparse, explicit) == explicit
