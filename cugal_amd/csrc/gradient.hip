@@ -111,6 +111,48 @@ __device__ __forceinline__ void st_stream4(float* p, float4 v) {
     __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(p));
 }
 
+// Sum of the segments X[col[k]][cq..cq+3] for k in [k0, k1) of the staged
+// index chunk (sidx[k - cs] = col[k]), in neighbour order; eight loads in
+// flight per step.
+__device__ __forceinline__ void gather_range(float4& acc, const int* __restrict__ sidx, int cs, int k0, int k1,
+                                             const float* __restrict__ X, int64_t ldx, int cq) {
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+        float4 x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = *reinterpret_cast<const float4*>(X + sidx[k - cs + q] * ldx + cq);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            acc.x += x[q].x;
+            acc.y += x[q].y;
+            acc.z += x[q].z;
+            acc.w += x[q].w;
+        }
+    }
+    for (; k + 4 <= k1; k += 4) {
+        float4 x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const float4*>(X + sidx[k - cs + q] * ldx + cq);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc.x += x[q].x;
+            acc.y += x[q].y;
+            acc.z += x[q].z;
+            acc.w += x[q].w;
+        }
+    }
+    for (; k < k1; ++k) {
+        const float4 x = *reinterpret_cast<const float4*>(X + sidx[k - cs] * ldx + cq);
+        acc.x += x.x;
+        acc.y += x.y;
+        acc.z += x.z;
+        acc.w += x.w;
+    }
+}
+
+// Column indices of a tile's rows are staged in LDS in chunks of this many.
+constexpr int kIdxChunk = 2048;
+
 template <bool EPI, bool VEC>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
@@ -118,73 +160,61 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
                                                            Epilogue ep, float* __restrict__ out, int64_t ldo,
                                                            int tiles_r, int64_t tiles) {
     __shared__ float tile[kTileR][kTileC + 1];
+    __shared__ int sidx[VEC ? kIdxChunk : 1];
+    __shared__ int srp[kTileR + 1];
     const int64_t t = xcd_tile(blockIdx.x, tiles);
     if (t >= tiles) return;
     const int r0 = static_cast<int>(t % tiles_r) * kTileR;
     const int c0 = static_cast<int>(t / tiles_r) * kTileC;
-    {
+    if (VEC) {
+        // The tile's index lists, staged through LDS with coalesced loads, so
+        // each gather step issues its row loads without a dependent index
+        // load first (measured 11 % faster per gradient at n = 32768).
         const int g = threadIdx.x >> 3;  // 32 row groups of 8 threads
         const int q = threadIdx.x & 7;   // float4 slot within a 32-float line
+        const int cq = c0 + 4 * q;
+        if (threadIdx.x <= kTileR) srp[threadIdx.x] = rp[min(r0 + static_cast<int>(threadIdx.x), n)];
+        __syncthreads();
+        const int e_lo = srp[0];
+        const int e_hi = srp[kTileR];
+        float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int cs = e_lo; cs < e_hi; cs += kIdxChunk) {
+            const int ce = min(e_hi, cs + kIdxChunk);
+            for (int k = cs + static_cast<int>(threadIdx.x); k < ce; k += kSpThreads) sidx[k - cs] = col[k];
+            __syncthreads();
+            if (cq < n) {
+                const int b0 = max(srp[g], cs), e0 = min(srp[g + 1], ce);
+                const int b1 = max(srp[g + 32], cs), e1 = min(srp[g + 33], ce);
+                // One row after the other: gathering both at once measured
+                // slower (more registers, fewer waves in flight).
+                gather_range(acc0, sidx, cs, b0, e0, X, ldx, cq);
+                gather_range(acc1, sidx, cs, b1, e1, X, ldx, cq);
+            }
+            __syncthreads();
+        }
+        tile[g][4 * q + 0] = acc0.x;
+        tile[g][4 * q + 1] = acc0.y;
+        tile[g][4 * q + 2] = acc0.z;
+        tile[g][4 * q + 3] = acc0.w;
+        tile[g + 32][4 * q + 0] = acc1.x;
+        tile[g + 32][4 * q + 1] = acc1.y;
+        tile[g + 32][4 * q + 2] = acc1.z;
+        tile[g + 32][4 * q + 3] = acc1.w;
+    } else {
+        const int g = threadIdx.x >> 3;
+        const int q = threadIdx.x & 7;
         const int cq = c0 + 4 * q;
         for (int rr = g; rr < kTileR; rr += 32) {
             const int r = r0 + rr;
             float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
             if (r < n) {
-                const int b = rp[r];
-                const int e = rp[r + 1];
-                if (VEC) {
-                    if (cq < n) {
-                        // Eight neighbours per step: their indices, then
-                        // their segments, are in flight together; the sum
-                        // keeps the neighbour order.
-                        int k = b;
-                        for (; k + 8 <= e; k += 8) {
-                            int64_t kk[8];
-                            float4 x[8];
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) kk[q] = col[k + q];
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) x[q] = *reinterpret_cast<const float4*>(X + kk[q] * ldx + cq);
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) {
-                                acc.x += x[q].x;
-                                acc.y += x[q].y;
-                                acc.z += x[q].z;
-                                acc.w += x[q].w;
-                            }
-                        }
-                        for (; k + 4 <= e; k += 4) {
-                            int64_t kk[4];
-                            float4 x[4];
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) kk[q] = col[k + q];
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const float4*>(X + kk[q] * ldx + cq);
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                acc.x += x[q].x;
-                                acc.y += x[q].y;
-                                acc.z += x[q].z;
-                                acc.w += x[q].w;
-                            }
-                        }
-                        for (; k < e; ++k) {
-                            const float4 x =
-                                *reinterpret_cast<const float4*>(X + static_cast<int64_t>(col[k]) * ldx + cq);
-                            acc.x += x.x;
-                            acc.y += x.y;
-                            acc.z += x.z;
-                            acc.w += x.w;
-                        }
-                    }
-                } else {
-                    for (int k = b; k < e; ++k) {
-                        const float* xr = X + static_cast<int64_t>(col[k]) * ldx;
-                        if (cq + 0 < n) acc.x += xr[cq + 0];
-                        if (cq + 1 < n) acc.y += xr[cq + 1];
-                        if (cq + 2 < n) acc.z += xr[cq + 2];
-                        if (cq + 3 < n) acc.w += xr[cq + 3];
-                    }
+                for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                    const float* xr = X + static_cast<int64_t>(col[k]) * ldx;
+                    if (cq + 0 < n) acc.x += xr[cq + 0];
+                    if (cq + 1 < n) acc.y += xr[cq + 1];
+                    if (cq + 2 < n) acc.z += xr[cq + 2];
+                    if (cq + 3 < n) acc.w += xr[cq + 3];
                 }
             }
             tile[rr][4 * q + 0] = acc.x;

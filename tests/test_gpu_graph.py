@@ -93,10 +93,17 @@ def test_distance_matches_direct_form(device):
 
 
 @pytest.mark.parametrize("sparse", [True, False])
-@pytest.mark.parametrize("n,seed", [(256, 1), (999, 2), (1024, 3)])
-def test_gradient_matches_oracle(device, sparse, n, seed):
-    ga = nx.erdos_renyi_graph(n, 0.02, seed=seed)
-    gb = nx.erdos_renyi_graph(n, 0.02, seed=seed + 100)
+@pytest.mark.parametrize("n,seed,p,hub", [(256, 1, 0.02, False), (999, 2, 0.02, False), (1024, 3, 0.02, False),
+                                          (512, 4, 0.25, False), (3000, 5, 0.002, True)])
+def test_gradient_matches_oracle(device, sparse, n, seed, p, hub):
+    # (512, p = 0.25): a 64-row tile lists ~8000 neighbours, several staged
+    # index chunks; hub: node 0 is joined to every node, so its row alone
+    # spans chunks.
+    ga = nx.erdos_renyi_graph(n, p, seed=seed)
+    gb = nx.erdos_renyi_graph(n, p, seed=seed + 100)
+    if hub:
+        ga.add_edges_from((0, j) for j in range(1, n))
+        gb.add_edges_from((n - 1, j) for j in range(n - 1))
     A = Adjacency.from_graph(ga, device)
     B = Adjacency.from_graph(gb, device)
     gen = torch.Generator().manual_seed(seed)
