@@ -44,11 +44,12 @@ _SIGNATURES = {
     "cg_add_distance": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_regularize": (c_int, [c_p, c_p, c_i64, c_int, c_p]),
     "cg_gradient_workspace": (c_sz, [c_i64]),
-    "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_gradient_padded_ld": (c_i64, [c_i64]),
+    "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
     "cg_adjacency_bits_words": (c_sz, [c_i64]),
     "cg_adjacency_pack_bits": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_gradient_mfma_workspace": (c_sz, [c_i64]),
-    "cg_gradient_mfma": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_gradient_mfma": (c_int, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
     "cg_sinkhorn_log_step": (c_int, [c_p, c_i64, c_p, c_p, c_f, c_p]),
     "cg_sinkhorn_log_marginal_workspace": (c_sz, [c_i64]),
     "cg_sinkhorn_log_marginal": (c_int, [c_p, c_i64, c_p, c_p, c_p, c_p, c_sz, c_p]),
@@ -72,8 +73,8 @@ _SIGNATURES = {
         c_int, [c_p, c_i64, c_f, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_f, c_p, c_p, c_sz, c_p],
     ),
     "cg_update_workspace": (c_sz, [c_i64]),
-    "cg_update_quasi_permutation_log": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
-    "cg_update_quasi_permutation_standard": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_update_quasi_permutation_log": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_update_quasi_permutation_standard": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
     "cg_lap_workspace": (c_sz, [c_i64]),
     "cg_lap_greedy": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_sz, c_p]),
     "cg_lap_greedy_random": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_p, c_p, c_sz, c_p]),
@@ -125,6 +126,14 @@ def query(name: str, *args):
 
 def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
     return None if t is None else t.data_ptr()
+
+
+def row_stride(t: torch.Tensor, name: str) -> int:
+    """Row stride (elements) of a 2-D tensor whose rows are contiguous, as
+    the entry points that take a row stride expect; raises otherwise."""
+    if t.dim() != 2 or t.stride(1) != 1 or t.stride(0) < t.shape[1]:
+        raise ValueError(f"{name} must be a 2-D tensor with contiguous rows, got strides {tuple(t.stride())}")
+    return int(t.stride(0))
 
 
 def require_device_tensor(t: torch.Tensor, name: str, dtype: torch.dtype, numel: Optional[int] = None) -> None:

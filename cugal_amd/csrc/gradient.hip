@@ -27,6 +27,7 @@ constexpr int kNumXcd = 8;
 // ---- row gather out[i][c] = scale * sum_{k in N(i)} X[k][c] (+ epilogue) --
 struct Epilogue {
     const float* P;   // nullptr: no regulariser term
+    int64_t ldp;      // row stride of P
     const float* fs;  // source features n x 4 (nullptr: no distance term)
     const float* ft;  // target features n x 4
     float rl;         // f * lambda
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         const int c = c0 + cc;
         if (c >= n) break;
         float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
-        const int64_t pbase = static_cast<int64_t>(c) * n;
+        const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
         const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
             const float4 fsi = ep.fs ? *reinterpret_cast<const float4*>(ep.fs + 4 * static_cast<int64_t>(c))
@@ -250,11 +251,11 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
     }
 }
 
-// out (row stride ldo) = X (row stride n): row-major streaming copy.
-__global__ __launch_bounds__(kSpThreads) void copy_rows_padded(const float* __restrict__ X, int n,
+// out (row stride ldo) = X (row stride ldx): row-major streaming copy.
+__global__ __launch_bounds__(kSpThreads) void copy_rows_padded(const float* __restrict__ X, int64_t ldx, int n,
                                                               float* __restrict__ out, int64_t ldo) {
     const int64_t r = blockIdx.x;
-    const float* src = X + r * n;
+    const float* src = X + r * ldx;
     float* dst = out + r * ldo;
     for (int j = threadIdx.x * 4; j < n; j += kSpThreads * 4) st_stream4(dst + j, ld_stream4(src + j));
 }
@@ -339,7 +340,7 @@ int cg_adjacency_matmul(const int32_t* col_indices, const int32_t* row_pointers,
                         float* out, int negate, cg_stream_t stream) {
     if (!col_indices || !row_pointers || !matrix || !out || bad_n(n))
         return fail(CG_EINVAL, "cg_adjacency_matmul: bad arguments");
-    Epilogue ep{nullptr, nullptr, nullptr, 0.0f};
+    Epilogue ep{nullptr, 0, nullptr, nullptr, 0.0f};
     return launch_rowgather(col_indices, row_pointers, static_cast<int>(n), matrix, negate ? -1.0f : 1.0f, ep, out,
                             reinterpret_cast<hipStream_t>(stream));
 }
@@ -363,7 +364,13 @@ int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_
 
 // Row stride of the intermediate: a power-of-two stride would map the lines
 // of one column strip onto the same few L2 sets.
-static int64_t gradient_ld(int64_t n) { return round_up(n, 4) + 64; }
+static int64_t gradient_ld(int64_t n) {
+    const int64_t ld = round_up(n, 4) + 64;
+    return (ld * 4) % 4096 == 0 ? ld + 32 : ld;
+}
+
+// A row stride whose 128-byte column segments spread over the L2 sets.
+static bool gather_friendly_ld(int64_t ld) { return ld % 4 == 0 && (ld * 4) % 4096 != 0; }
 
 // Two padded n x n buffers: a copy of P (gathered by the first product) and
 // the intermediate Y.
@@ -371,11 +378,13 @@ size_t cg_gradient_workspace(int64_t n) {
     return n > 0 ? 2 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) : 0;
 }
 
+int64_t cg_gradient_padded_ld(int64_t n) { return n > 0 ? gradient_ld(n) : 0; }
+
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
-                int64_t n, const float* P, const float* source_features, const float* target_features, int lambda_,
-                float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
+                int lambda_, float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                 cg_stream_t stream) {
-    if (!a_col || !a_rowptr || !b_col || !b_rowptr || !P || !gradient || bad_n(n))
+    if (!a_col || !a_rowptr || !b_col || !b_rowptr || !P || !gradient || bad_n(n) || ld_p < n)
         return fail(CG_EINVAL, "cg_gradient: bad arguments");
     if ((source_features == nullptr) != (target_features == nullptr))
         return fail(CG_EINVAL, "cg_gradient: give both feature matrices or neither");
@@ -389,18 +398,19 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
     float* Pp = cv.take<float>(static_cast<size_t>(n) * ldy);
     float* Y = cv.take<float>(static_cast<size_t>(n) * ldy);
-    const Epilogue none{nullptr, nullptr, nullptr, 0.0f};
+    const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f};
     const float* X1 = P;
-    int64_t ld1 = n;
-    if (n % 4 == 0 && aligned16(P)) {
-        hipLaunchKernelGGL(copy_rows_padded, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, P, ni, Pp, ldy);
+    int64_t ld1 = ld_p;
+    if (n % 4 == 0 && aligned16(P) && !gather_friendly_ld(ld_p)) {
+        hipLaunchKernelGGL(copy_rows_padded, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, P, ld_p, ni,
+                           Pp, ldy);
         if (int rc = check_launch("copy_rows_padded")) return rc;
         X1 = Pp;
         ld1 = ldy;
     }
     if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
-    Epilogue ep{rl != 0.0f ? P : nullptr, source_features, target_features, rl};
+    Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl};
     return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
 }
 

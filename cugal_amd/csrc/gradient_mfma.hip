@@ -95,7 +95,7 @@ __device__ __forceinline__ bf16x8 expand_bits(uint32_t word, int chunk) {
 }
 
 // ---- 1. PT_hi / PT_lo = split(P^T), zero padded --------------------------
-__global__ __launch_bounds__(256) void split_transpose(const float* __restrict__ P, int n, int64_t np,
+__global__ __launch_bounds__(256) void split_transpose(const float* __restrict__ P, int64_t ldp, int n, int64_t np,
                                                       uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
     __shared__ float tile[64][65];
     const int64_t r0 = static_cast<int64_t>(blockIdx.y) * 64;  // rows of P
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void split_transpose(const float* __restrict__
     const int ty = threadIdx.x >> 6;
     for (int i = ty; i < 64; i += 4) {
         const int64_t r = r0 + i, c = c0 + tx;
-        tile[i][tx] = (r < n && c < n) ? P[r * n + c] : 0.0f;
+        tile[i][tx] = (r < n && c < n) ? P[r * ldp + c] : 0.0f;
     }
     __syncthreads();
     for (int j = ty; j < 64; j += 4) {
@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void split_transpose(const float* __restrict__
 
 struct GradEpi {
     const float* P;   // nullptr: no regulariser term
+    int64_t ldp;      // row stride of P
     const float* fs;  // source features n x 4 (nullptr: no distance term)
     const float* ft;
     float rl;         // f * lambda
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
                         const float d0 = fsm.x - b.x, d1 = fsm.y - b.y, d2 = fsm.z - b.z, d3 = fsm.w - b.w;
                         g += sqrtf(fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3))));
                     }
-                    if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[m * n + c];
+                    if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[m * ep.ldp + c];
                     ep.G[m * n + c] = g;
                 }
             }
@@ -388,11 +389,11 @@ size_t cg_gradient_mfma_workspace(int64_t n) {
     return cv.used + 256;
 }
 
-int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P,
+int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P, int64_t ld_p,
                      const float* source_features, const float* target_features, int lambda_,
                      float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                      cg_stream_t stream) {
-    if (!a_bits || !b_bits || !P || !gradient || n <= 0 || n > (int64_t(1) << 20))
+    if (!a_bits || !b_bits || !P || !gradient || n <= 0 || n > (int64_t(1) << 20) || ld_p < n)
         return fail(CG_EINVAL, "cg_gradient_mfma: bad arguments");
     if ((source_features == nullptr) != (target_features == nullptr))
         return fail(CG_EINVAL, "cg_gradient_mfma: give both feature matrices or neither");
@@ -412,12 +413,12 @@ int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, 
     uint16_t* y_hi = cv.take<uint16_t>(e);
     uint16_t* y_lo = cv.take<uint16_t>(e);
     const unsigned tb = static_cast<unsigned>(np / 64);
-    hipLaunchKernelGGL(split_transpose, dim3(tb, tb), dim3(256), 0, s, P, ni, np, pt_hi, pt_lo);
+    hipLaunchKernelGGL(split_transpose, dim3(tb, tb), dim3(256), 0, s, P, ld_p, ni, np, pt_hi, pt_lo);
     if (int rc = check_launch("split_transpose")) return rc;
-    GradEpi ep1{nullptr, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo};
+    GradEpi ep1{nullptr, 0, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo};
     if (int rc = launch_gemm<false, 0>(a_bits, pt_hi, pt_lo, ni, np, 1.0f, ep1, s, "gemm_nt<Y = A P>")) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
-    GradEpi ep2{rl != 0.0f ? P : nullptr, source_features, target_features, rl, gradient, nullptr, nullptr};
+    GradEpi ep2{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, gradient, nullptr, nullptr};
     return launch_gemm<true, 1>(b_bits, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
 }
 

@@ -22,7 +22,8 @@ __device__ __forceinline__ float plan_of(float c, float neg_rinv, float ui, floa
 }
 
 template <bool VEC, bool STD>
-__global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, const float* __restrict__ C, int n,
+__global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, int64_t ldp,
+                                                        const float* __restrict__ C, int n,
                                                         const float* __restrict__ u, const float* __restrict__ v,
                                                         float alpha, float neg_rinv, int rows_per_blk,
                                                         double* __restrict__ part) {
@@ -32,13 +33,14 @@ __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, 
     double gap = 0.0;
     for (int i = r0; i < r1; ++i) {
         const int64_t base = static_cast<int64_t>(i) * n;
+        float* __restrict__ prow = P + static_cast<int64_t>(i) * ldp;
         const float ui = u[i];
         float rowgap = 0.0f;
         if (VEC) {
             for (int j = threadIdx.x * 4; j < n; j += kUpdThreads * 4) {
                 const float4 c = ld_stream4(C + base + j);
                 const float4 vv = *reinterpret_cast<const float4*>(v + j);
-                float4 p = *reinterpret_cast<const float4*>(P + base + j);
+                float4 p = *reinterpret_cast<const float4*>(prow + j);
                 const float e0 = plan_of<STD>(c.x, neg_rinv, ui, vv.x) - p.x;
                 const float e1 = plan_of<STD>(c.y, neg_rinv, ui, vv.y) - p.y;
                 const float e2 = plan_of<STD>(c.z, neg_rinv, ui, vv.z) - p.z;
@@ -48,15 +50,15 @@ __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, 
                 p.y = fmaf(alpha, e1, p.y);
                 p.z = fmaf(alpha, e2, p.z);
                 p.w = fmaf(alpha, e3, p.w);
-                *reinterpret_cast<float4*>(P + base + j) = p;
+                *reinterpret_cast<float4*>(prow + j) = p;
             }
         } else {
             for (int j = threadIdx.x; j < n; j += kUpdThreads) {
                 const float c = C[base + j];
-                const float p = P[base + j];
+                const float p = prow[j];
                 const float d = plan_of<STD>(c, neg_rinv, ui, v[j]) - p;
                 rowgap += d * c;
-                P[base + j] = fmaf(alpha, d, p);
+                prow[j] = fmaf(alpha, d, p);
             }
         }
         gap += static_cast<double>(rowgap);
@@ -94,30 +96,31 @@ size_t cg_update_workspace(int64_t n) {
 
 namespace {
 
-int run_update(bool standard, float* P, const float* C, int64_t n, const float* u, const float* v, float alpha,
+int run_update(bool standard, float* P, int64_t ldp, const float* C, int64_t n, const float* u, const float* v,
+               float alpha,
                float regularization, double* gap_device, void* workspace, size_t workspace_bytes,
                hipStream_t s, const char* who) {
-    if (!P || !C || !u || !v || !gap_device || n <= 0 || n > (int64_t(1) << 30))
+    if (!P || !C || !u || !v || !gap_device || n <= 0 || n > (int64_t(1) << 30) || ldp < n)
         return fail(CG_EINVAL, "%s: bad arguments", who);
     if (!(regularization > 0.0f)) return fail(CG_EINVAL, "regularization must be > 0");
     if (!workspace || workspace_bytes < cg_update_workspace(n)) return fail(CG_EWORKSPACE, "%s: workspace too small", who);
     int rpb = 0;
     const int blocks = update_blocks(n, &rpb);
     double* part = static_cast<double*>(workspace);
-    const bool vec = (n % 4 == 0) && aligned16(P) && aligned16(C) && aligned16(v);
+    const bool vec = (n % 4 == 0) && (ldp % 4 == 0) && aligned16(P) && aligned16(C) && aligned16(v);
     const float neg_rinv = -1.0f / regularization;
     const int ni = static_cast<int>(n);
     if (vec && standard)
-        hipLaunchKernelGGL((fw_update<true, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+        hipLaunchKernelGGL((fw_update<true, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, ni, u, v, alpha,
                            neg_rinv, rpb, part);
     else if (vec)
-        hipLaunchKernelGGL((fw_update<true, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+        hipLaunchKernelGGL((fw_update<true, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, ni, u, v, alpha,
                            neg_rinv, rpb, part);
     else if (standard)
-        hipLaunchKernelGGL((fw_update<false, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+        hipLaunchKernelGGL((fw_update<false, true>), dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, ni, u, v, alpha,
                            neg_rinv, rpb, part);
     else
-        hipLaunchKernelGGL((fw_update<false, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, C, ni, u, v, alpha,
+        hipLaunchKernelGGL((fw_update<false, false>), dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, ni, u, v, alpha,
                            neg_rinv, rpb, part);
     if (int rc = check_launch("fw_update")) return rc;
     hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, part, blocks, gap_device);
@@ -126,17 +129,17 @@ int run_update(bool standard, float* P, const float* C, int64_t n, const float* 
 
 }  // namespace
 
-int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
-                                    float alpha, float regularization, double* gap_device, void* workspace,
-                                    size_t workspace_bytes, cg_stream_t stream) {
-    return run_update(false, P, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
+int cg_update_quasi_permutation_log(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                    const float* v, float alpha, float regularization, double* gap_device,
+                                    void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    return run_update(false, P, ld_p, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
                       reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_log");
 }
 
-int cg_update_quasi_permutation_standard(float* P, const float* C, int64_t n, const float* u, const float* v,
-                                         float alpha, float regularization, double* gap_device, void* workspace,
-                                         size_t workspace_bytes, cg_stream_t stream) {
-    return run_update(true, P, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
+int cg_update_quasi_permutation_standard(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                         const float* v, float alpha, float regularization, double* gap_device,
+                                         void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    return run_update(true, P, ld_p, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
                       reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_standard");
 }
 

@@ -73,6 +73,9 @@ def _gradient_into(
     accumulation (cg_gradient_mfma)."""
     n = P.shape[0]
     dev = P.device
+    if P.dim() != 2 or P.stride(1) != 1:
+        P = P.contiguous()
+    ldp = nat.row_stride(P, "P")
     fs = ft = None
     if isinstance(features, Features):
         fs, ft = features.source.contiguous(), features.target.contiguous()
@@ -80,14 +83,14 @@ def _gradient_into(
     if precision == "bf16x2":
         ws = nat.workspace(dev, nat.query("cg_gradient_mfma_workspace", n), "gradient_mfma")
         nat.call(
-            "cg_gradient_mfma", nat.ptr(A.packed_bits()), nat.ptr(B.packed_bits()), n, nat.ptr(P), nat.ptr(fs),
+            "cg_gradient_mfma", nat.ptr(A.packed_bits()), nat.ptr(B.packed_bits()), n, nat.ptr(P), ldp, nat.ptr(fs),
             nat.ptr(ft), int(iteration), float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
         )
     elif precision == "fp32":
         ws = nat.workspace(dev, nat.query("cg_gradient_workspace", n), "gradient")
         nat.call(
             "cg_gradient", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(B.col_indices),
-            nat.ptr(B.row_pointers), n, nat.ptr(P), nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
+            nat.ptr(B.row_pointers), n, nat.ptr(P), ldp, nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
             nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
         )
     else:
@@ -140,8 +143,9 @@ def update_quasi_permutation_log(P: torch.Tensor, K: LogKernel, u: torch.Tensor,
     out = torch.empty(1, dtype=torch.float64, device=dev) if gap is None else gap
     ws = nat.workspace(dev, nat.query("cg_update_workspace", n), "update")
     nat.call(
-        "cg_update_quasi_permutation_log", nat.ptr(P), nat.ptr(K.cost), n, nat.ptr(u), nat.ptr(v), float(alpha),
-        float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+        "cg_update_quasi_permutation_log", nat.ptr(P), nat.row_stride(P, "P"), nat.ptr(K.cost), n, nat.ptr(u),
+        nat.ptr(v), float(alpha), float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(),
+        nat.stream_of(dev),
     )
     return abs(float(out.item())) if gap is None else None
 
@@ -156,8 +160,9 @@ def update_quasi_permutation_standard(P: torch.Tensor, K: StandardKernel, u: tor
     out = torch.empty(1, dtype=torch.float64, device=dev) if gap is None else gap
     ws = nat.workspace(dev, nat.query("cg_update_workspace", n), "update")
     nat.call(
-        "cg_update_quasi_permutation_standard", nat.ptr(P), nat.ptr(K.cost), n, nat.ptr(u), nat.ptr(v),
-        float(alpha), float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+        "cg_update_quasi_permutation_standard", nat.ptr(P), nat.row_stride(P, "P"), nat.ptr(K.cost), n,
+        nat.ptr(u), nat.ptr(v), float(alpha), float(K.regularization), nat.ptr(out), nat.ptr(ws), ws.numel(),
+        nat.stream_of(dev),
     )
     return abs(float(out.item())) if gap is None else None
 
@@ -202,7 +207,12 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
         raise ValueError(f"graphs differ in size ({n} vs {B.number_of_nodes()})")
     dev = A.row_pointers.device
     sinkhorn_state = SinkhornState(n, config)
-    P = torch.full((n, n), 1.0 / n, device=dev, dtype=torch.float32)
+    # P lives in a row-padded buffer during the loop: the gradient gathers
+    # its rows in place (a 2^k-float row stride would alias L2 sets and cost
+    # a copy per step); the caller gets a contiguous copy.
+    ldp = int(nat.query("cg_gradient_padded_ld", n))
+    P = torch.empty((n, ldp), device=dev, dtype=torch.float32)[:, :n]
+    P.fill_(1.0 / n)
     gradient = torch.empty((n, n), device=dev, dtype=torch.float32)
     gap = torch.empty(1, dtype=torch.float64, device=dev)
     factor = 1.0 if config.use_sparse_adjacency else 2.0
@@ -227,7 +237,7 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
             profile.frank_wolfe_iterations += 1
             if config.frank_wolfe_threshold is not None and abs(float(gap.item())) < config.frank_wolfe_threshold:
                 break
-    return P
+    return P.contiguous()
 
 
 def hungarian(quasi_permutation: torch.Tensor, config: Config, profile: Profile):

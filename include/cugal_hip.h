@@ -116,10 +116,16 @@ int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_
  * on the fly (no n x n distance matrix; features may be NULL for D = 0).
  * regularizer_factor = 1 is sparse_gradient (cugal/pred.py:54-74); 2 is
  * dense_gradient, which adds lambda (1 - 2P) twice (cugal/pred.py:46-50).
- * Workspace: one n x n FP32 intermediate (P B). */
+ * ld_p: row stride of P in floats (>= n; n for a contiguous P).  The first
+ * product gathers 128-byte segments of P's rows; with a stride that is a
+ * multiple of 4 KiB those alias a few L2 sets, so such a P is first copied
+ * to a row-padded buffer, while any other stride (cg_gradient_padded_ld)
+ * is gathered in place.  Workspace: the padded copy and the intermediate
+ * (A P)^T, n x (n + 64) FP32 each. */
 size_t cg_gradient_workspace(int64_t n);
+int64_t cg_gradient_padded_ld(int64_t n);
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
-                int64_t n, const float* P, const float* source_features, const float* target_features,
+                int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
                 int lambda_, float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                 cg_stream_t stream);
 
@@ -229,7 +235,7 @@ size_t cg_adjacency_bits_words(int64_t n);
 int cg_adjacency_pack_bits(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint32_t* out,
                            cg_stream_t stream);
 size_t cg_gradient_mfma_workspace(int64_t n);
-int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P,
+int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P, int64_t ld_p,
                      const float* source_features, const float* target_features, int lambda_,
                      float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                      cg_stream_t stream);
@@ -243,18 +249,19 @@ int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, 
  * cugal/pred.py:134-136): with plan = exp(-C/reg + u_i + v_j),
  *   gap = sum (plan - P) * C  (FP64),   P += alpha * (plan - P).
  * *gap_device (a device double) receives the signed sum; the reference
- * reports |gap|.  Takes the cost C instead of K = -C/reg. */
+ * reports |gap|.  Takes the cost C instead of K = -C/reg (contiguous); P has
+ * row stride ld_p >= n. */
 size_t cg_update_workspace(int64_t n);
-int cg_update_quasi_permutation_log(float* P, const float* C, int64_t n, const float* u, const float* v,
-                                    float alpha, float regularization, double* gap_device, void* workspace,
-                                    size_t workspace_bytes, cg_stream_t stream);
+int cg_update_quasi_permutation_log(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                    const float* v, float alpha, float regularization, double* gap_device,
+                                    void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 /* The STANDARD form of the same step (cugal/pred.py:77-86 with
  * scale_kernel_matrix, cugal/sinkhorn.py:20-23; gap as :139-141):
  * plan = u_i exp(-C/reg) v_j with the scalings of cg_sinkhorn_standard_solve. */
-int cg_update_quasi_permutation_standard(float* P, const float* C, int64_t n, const float* u, const float* v,
-                                         float alpha, float regularization, double* gap_device, void* workspace,
-                                         size_t workspace_bytes, cg_stream_t stream);
+int cg_update_quasi_permutation_standard(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                         const float* v, float alpha, float regularization, double* gap_device,
+                                         void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Rounding to a permutation (LAP).

@@ -126,6 +126,32 @@ def test_gradient_matches_oracle(device, sparse, n, seed, p, hub):
     np.testing.assert_allclose(out.double().cpu().numpy(), ref.numpy(), rtol=0, atol=1e-5 * scale)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
+@pytest.mark.parametrize("n", [1024, 1000])
+def test_gradient_of_row_padded_P(device, precision, n):
+    # The gradient of P given as a view with padded rows (gathered in place)
+    # equals the gradient of the contiguous P bit for bit; a transposed view
+    # is the matrix it shows.
+    ga = nx.watts_strogatz_graph(n, 8, 0.2, seed=1)
+    gb = nx.barabasi_albert_graph(n, 4, seed=2)
+    A = Adjacency.from_graph(ga, device)
+    B = Adjacency.from_graph(gb, device)
+    cfg = Config(device="cuda", dtype=torch.float32, mu=0.5)
+    feats = Features(node_features(A, cfg.mu), node_features(B, cfg.mu))
+    P = torch.rand((n, n), generator=torch.Generator(device=device).manual_seed(3), device=device) / n
+    ld = int(nat.query("cg_gradient_padded_ld", n))
+    Pv = torch.empty((n, ld), device=device)[:, :n]
+    Pv.copy_(P)
+    want = torch.empty((n, n), device=device)
+    got = torch.empty((n, n), device=device)
+    _gradient_into(want, A, B, P, feats, 3, 1.0, precision)
+    _gradient_into(got, A, B, Pv, feats, 3, 1.0, precision)
+    assert torch.equal(got, want)
+    _gradient_into(want, A, B, P.T.contiguous(), feats, 3, 1.0, precision)
+    _gradient_into(got, A, B, P.T, feats, 3, 1.0, precision)
+    assert torch.equal(got, want)
+
+
 def test_adjacency_mul_matches_dense(device):
     n = 700
     g = nx.barabasi_albert_graph(n, 4, seed=9)

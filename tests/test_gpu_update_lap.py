@@ -35,6 +35,28 @@ def test_update_matches_oracle(device, n):
     assert abs(abs(float(gap.item())) - want_gap) <= 1e-4 * want_gap + 1e-6
 
 
+@pytest.mark.parametrize("n,pad", [(1000, 64), (1024, 96), (777, 3)])
+def test_update_of_row_padded_P(device, n, pad):
+    # P as the n x n view of an n x (n + pad) buffer: the same P and gap as a
+    # contiguous P, and the padding columns are left alone.
+    g = torch.Generator().manual_seed(n)
+    C = torch.rand((n, n), generator=g).to(device) * 5
+    u = (torch.randn(n, generator=g) * 0.5).to(device)
+    v = (torch.randn(n, generator=g) * 0.5 - 3.0).to(device)
+    P = (torch.rand((n, n), generator=g) / n).to(device)
+    buf = torch.full((n, n + pad), -7.0, device=device)
+    Pv = buf[:, :n]
+    Pv.copy_(P)
+    gaps = torch.empty(2, dtype=torch.float64, device=device)
+    update_quasi_permutation_log(P, LogKernel(C, 0.7), u, v, 0.4, gap=gaps[0:1])
+    update_quasi_permutation_log(Pv, LogKernel(C, 0.7), u, v, 0.4, gap=gaps[1:2])
+    assert torch.equal(P, Pv)
+    assert float(gaps[0]) == float(gaps[1])
+    assert bool((buf[:, n:] == -7.0).all())
+    with pytest.raises(ValueError):
+        update_quasi_permutation_log(P.T, LogKernel(C, 0.7), u, v, 0.4, gap=gaps[0:1])
+
+
 def test_update_alpha_one_gives_plan(device):
     n = 256
     g = torch.Generator().manual_seed(1)
