@@ -202,6 +202,7 @@ def probe(args) -> int:
     from cugal_amd.feature_extraction import Features, node_features
     from cugal_amd.greedy_lap import greedy_lap_device
     from cugal_amd.hungarian import dense_hungarian
+    from cugal_amd import _native as nat
     from cugal_amd.pred import _gradient_into, update_quasi_permutation_log, update_quasi_permutation_standard
     from cugal_amd.sinkhorn import LogKernel, StandardKernel, solve_log_native, solve_standard_native
     from cugal_amd.synthetic import watts_strogatz
@@ -213,9 +214,11 @@ def probe(args) -> int:
     B = Adjacency.from_graph(inst.target, dev)
     feats = Features(node_features(A, args.mu), node_features(B, args.mu))
     g = torch.Generator(device=dev).manual_seed(0)
-    P = torch.rand((n, n), generator=g, device=dev)
+    # P row-padded, as the alignment keeps it (the gradient gathers it in place)
+    P = torch.empty((n, int(nat.query("cg_gradient_padded_ld", n))), device=dev)[:, :n]
+    P.copy_(torch.rand((n, n), generator=g, device=dev))
     P /= P.sum(1, keepdim=True)
-    G = torch.empty_like(P)
+    G = torch.empty((n, n), device=dev)
     for lam in (3, 4):
         _gradient_into(G, A, B, P, feats, lam, 1.0, precision="bf16x2")
     for lam in (3, 4):
@@ -244,8 +247,8 @@ def probe(args) -> int:
     Q[swap, perm[torch.roll(swap, 1)]] += 0.6
     dense_hungarian(Q, maximize=True)
     Q = None
-    greedy_lap_device(P, HungarianMethod.DOUBLE_GREEDY)
-    copy = torch.empty_like(P)
+    greedy_lap_device(P.contiguous(), HungarianMethod.DOUBLE_GREEDY)
+    copy = torch.empty_like(G)
     for _ in range(3):
         copy.copy_(G)
     torch.cuda.synchronize()
