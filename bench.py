@@ -162,6 +162,87 @@ def gradient_error_sample(A, B, P, rows: int = 16, seed: int = 0):
             "max_err_vs_fp32_csr": vs_fp32, "norm": "max |error| / max |reference| of the row or matrix"}
 
 
+def _torch_unrolled_plan(C, reg: float, iterations: int):
+    """Plain-PyTorch FP64 unrolled log-Sinkhorn (v = -LSE_i(K + u), then
+    u = -LSE_j(K + v), K = -C/reg, from zero duals), differentiated by torch
+    autograd: the reference the hand-written adjoint is checked against."""
+    import torch
+
+    K = -C / reg
+    u = torch.zeros(C.shape[0], dtype=C.dtype, device=C.device)
+    v = torch.zeros_like(u)
+    for _ in range(iterations):
+        v = -torch.logsumexp(K + u[:, None], dim=0)
+        u = -torch.logsumexp(K + v[None, :], dim=1)
+    return torch.exp(K + u[:, None] + v[None, :])
+
+
+def differentiation_form(seed: int = 0, repeats: int = 5):
+    """Erdos-Renyi n=4096, p=0.004, 3% edge noise; cost lam * D(w) over
+    degree, clustering, mean neighbour degree and ego-net edges with
+    w ~ U(0.5, 1.5); 100 unrolled log-Sinkhorn iterations at eps = 0.01 with
+    the duals checkpointed every 10; loss = minus the plan mass on the
+    ground-truth mapping.  Times the forward (cost build + iterations + plan)
+    and the hand-written backward with HIP events on the launch stream
+    (median of `repeats`), and states the error of dL/dw, dL/dlam and the
+    plan against torch autograd through an FP64 unrolled Sinkhorn."""
+    import torch
+
+    from cugal_amd.adjacency import Adjacency
+    from cugal_amd.sinkhorn_grad import feature_sinkhorn_plan, structural_features
+    from cugal_amd.synthetic import erdos_renyi
+
+    dev = torch.device("cuda:0")
+    inst = erdos_renyi(4096, 0.004, 0.03, seed=seed)
+    src, tgt = inst.source, inst.target
+    n = max(max(src.nodes()), max(tgt.nodes())) + 1
+    src.add_nodes_from(range(n))
+    tgt.add_nodes_from(range(n))
+    fs = structural_features(Adjacency.from_graph(src, dev))
+    ft = structural_features(Adjacency.from_graph(tgt, dev))
+    w = 0.5 + torch.rand(4, generator=torch.Generator().manual_seed(seed + 11))
+    lam, reg, T, k = 0.05, 0.01, 100, 10
+    G = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    G[torch.arange(n, device=dev), torch.as_tensor(inst.source_mapping, device=dev)] = -1.0
+    fwd, bwd = [], []
+    for r in range(repeats + 1):
+        wg = w.to(dev).requires_grad_(True)
+        lg = torch.tensor(lam, dtype=torch.float64, device=dev, requires_grad=True)
+        start, mid, end = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        torch.cuda.synchronize()
+        start.record()
+        plan = feature_sinkhorn_plan(fs, ft, wg, lg, reg, T, k)
+        mid.record()
+        (plan * G).sum().backward()
+        end.record()
+        torch.cuda.synchronize()
+        if r > 0:  # the first run loads the kernels
+            fwd.append(start.elapsed_time(mid))
+            bwd.append(mid.elapsed_time(end))
+    fwd.sort()
+    bwd.sort()
+    fwd_ms, bwd_ms = fwd[len(fwd) // 2], bwd[len(bwd) // 2]
+    w64 = w.double().to(dev).requires_grad_(True)
+    l64 = torch.tensor(lam, dtype=torch.float64, device=dev, requires_grad=True)
+    d = (fs.double()[:, None, :] - ft.double()[None, :, :]).abs()
+    ref = _torch_unrolled_plan(l64 * (d * w64).sum(-1), reg, T)
+    d = None
+    (ref * G.double()).sum().backward()
+    plan_err = float((plan.detach().double() - ref.detach()).abs().max() / ref.detach().abs().max())
+    ref = None
+    rel_w = float((wg.grad.double() - w64.grad).abs().max() / w64.grad.abs().max())
+    rel_l = abs(float(lg.grad) - float(l64.grad)) / abs(float(l64.grad))
+    return {
+        "workload": f"erdos_renyi n={n} p=0.004, 3% edge noise; cost lam*D(w) over 4 structural features, "
+                    f"lam {lam}, eps {reg}, {T} unrolled iterations, checkpoints every {k}",
+        "n": n, "forward_ms": fwd_ms, "backward_ms": bwd_ms, "backward_over_forward": bwd_ms / fwd_ms,
+        "repeats": repeats, "grad_w": wg.grad.tolist(), "grad_w_fp64": w64.grad.tolist(),
+        "grad_lam": float(lg.grad), "grad_lam_fp64": float(l64.grad),
+        "max_rel_err_grad_w": rel_w, "rel_err_grad_lam": rel_l, "max_rel_err_plan": plan_err,
+        "reference": "torch autograd through an FP64 unrolled log-Sinkhorn on the device",
+    }
+
+
 def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: int):
     """FP64 oracle log-Sinkhorn (torch.logsumexp, cugal/sinkhorn.py:107-111)
     on a cost of the bench's size; times whole iterations until the budget
@@ -273,7 +354,8 @@ def main() -> int:
     ap.add_argument("--sinkhorn-iterations", type=int, default=500)
     ap.add_argument("--iter-count", type=int, default=15)
     ap.add_argument("--fw-iter-count", type=int, default=10)
-    ap.add_argument("--workload", choices=("ws32768", "ba8192", "sbm65536"), default="ws32768")
+    ap.add_argument("--workload", choices=("ws32768", "ba8192", "sbm65536", "er4096grad"), default="ws32768")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form (backward) run")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain two-sweep comparison run")
     ap.add_argument("--no-precision-form", action="store_true", help="skip the matrix-core gradient run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -296,6 +378,12 @@ def main() -> int:
         return run_ba8192(args)
     if args.workload == "sbm65536":
         return run_sbm65536(args)
+    if args.workload == "er4096grad":
+        diff = differentiation_form(args.seed)
+        log(f"differentiation form: forward {diff['forward_ms']:.1f} ms, backward {diff['backward_ms']:.1f} ms")
+        print(_contract_line(diff["backward_ms"] * 1e-3, args, diff["workload"] + " (value = backward seconds)",
+                             diff["n"], {"differentiation_form": diff}), flush=True)
+        return 0
     log(f"device {torch.cuda.get_device_name(0)}; generating WS(n={args.n}, k={args.k}, p={args.p})")
     t = time.perf_counter()
     inst = watts_strogatz(args.n, args.k, args.p, args.noise, seed=args.seed)
@@ -356,6 +444,12 @@ def main() -> int:
         log(f"precision form (MFMA gradient): {e:.2f}s acc {a:.4f}, gradient {grad_s:.2f}s "
             f"({issued:.0f} TFLOP/s issued)")
 
+    diff = None
+    if not args.no_diff:
+        diff = differentiation_form(args.seed)
+        log(f"differentiation form: forward {diff['forward_ms']:.1f} ms, backward {diff['backward_ms']:.1f} ms "
+            f"(x{diff['backward_over_forward']:.2f}); dL/dw rel err {diff['max_rel_err_grad_w']:.2e}")
+
     n = args.n
     avg_sweep_ms = sweep_ms / max(1, sweep_n)
     sweep_bytes = 4.0 * n * n
@@ -389,7 +483,7 @@ def main() -> int:
         "accuracy": sum(accs) / len(accs), "sinkhorn_iterations": iters_total,
         "sinkhorn_gbs": achieved, "phase_times": phase, "api_extra_s": api_extra, "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
-        "precision_form": precision,
+        "precision_form": precision, "differentiation_form": diff,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

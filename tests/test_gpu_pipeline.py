@@ -45,6 +45,27 @@ def test_pipeline_accuracy_matches_oracle(device, sparse, fused):
     assert all(abs(a - b) <= 10 for a, b in zip(got_iters, run.sinkhorn_iterations))
 
 
+def test_reference_default_config_er1000(device):
+    # Erdos-Renyi G(1000, 0.01), seed 0, 5% of edges removed and as many
+    # non-edges added, seeded permutation; the reference's default Config
+    # (cugal/config.py:59-120: STANDARD, reg 1.0, <= 500 iterations, thr
+    # 1e-3, mu 0.5, 15 x 10 FW, dense adjacency with the doubled lambda term,
+    # DOUBLE_GREEDY) on the device in FP32 against the FP64 oracle: node
+    # accuracy within 0.5 pp.
+    inst = erdos_renyi(1000, 0.01, 0.05, seed=0)
+    gpu = Config(device="cuda:0", dtype=torch.float32)
+    assert gpu.sinkhorn_method == SinkhornMethod.STANDARD and not gpu.use_sparse_adjacency
+    profile = Profile()
+    _, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, profile)
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), oracle.OracleConfig())
+    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    ref_acc = alignment_accuracy(inst.source_mapping, answer_columns(run.mapping))
+    print(f"\nER(1000) default config: accuracy {acc:.4f} vs oracle {ref_acc:.4f}; Sinkhorn iterations "
+          f"{sum(p.iteration_count for p in profile.sinkhorn_profiles)} vs {sum(run.sinkhorn_iterations)}")
+    assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
+    assert profile.frank_wolfe_iterations == 150
+
+
 def test_isomorphic_pair_recovered(device):
     inst = watts_strogatz(600, 10, 0.2, 0.0, seed=2)
     gpu, _ = _configs(True, 0.3, 2.0, 6, 6)

@@ -33,6 +33,51 @@ def test_grad_cost_matches_torch_autograd(device, k):
     np.testing.assert_allclose(g.numpy(), r.numpy(), rtol=0, atol=2e-3 * scale)
 
 
+def _grad_vs_autograd(device, C, reg, T, k, offset=0, seed=2):
+    n = C.shape[0]
+    M = _rand((n, n), seed) - 0.5
+    if offset:
+        # A cost whose data pointer is not 16-byte aligned takes the separate
+        # row and column passes instead of the fused adjoint sweep.
+        store = torch.empty(n * n + offset, device=device)
+        C_gpu = store[offset:].view(n, n)
+        C_gpu.copy_(C.to(device))
+        assert C_gpu.data_ptr() % 16 != 0
+        C_gpu.requires_grad_(True)
+    else:
+        C_gpu = C.to(device).requires_grad_(True)
+    plan = sinkhorn_plan(C_gpu, reg, T, checkpoint_every=k)
+    (plan.double() * M.to(device)).sum().backward()
+    C64 = C.double().to(device).requires_grad_(True)
+    ref_plan = oracle.unrolled_plan(C64, reg, T)
+    (ref_plan * M.to(device)).sum().backward()
+    np.testing.assert_allclose(plan.detach().double().cpu().numpy(), ref_plan.detach().cpu().numpy(), rtol=1e-4,
+                               atol=1e-8)
+    g, r = C_gpu.grad.double().cpu(), C64.grad.cpu()
+    scale = float(r.abs().max())
+    np.testing.assert_allclose(g.numpy(), r.numpy(), rtol=0, atol=2e-3 * scale)
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+@pytest.mark.parametrize("n,k,T", [(256, 4, 9), (1024, 10, 20)])
+def test_grad_fused_and_separate_passes(device, n, k, T, offset):
+    C = _rand((n, n), 21 + n, 3.0).float()
+    _grad_vs_autograd(device, C, 0.4, T, k, offset)
+
+
+def test_grad_longest_segment(device):
+    # 64 iterations per segment: the gradient pass holds (4 * 64 + 1) * 64
+    # floats of LDS (above the 64 KiB a launch gets without asking).
+    C = _rand((128, 128), 31, 2.0).float()
+    _grad_vs_autograd(device, C, 0.5, 70, 64)
+
+
+def test_grad_above_fused_size(device):
+    # n > 8192: the column vectors no longer fit in LDS; separate passes.
+    C = _rand((8200, 8200), 41, 3.0).float()
+    _grad_vs_autograd(device, C, 0.5, 4, 3)
+
+
 def test_grad_zero_iterations_is_plan_derivative(device):
     # plan = exp(-C/reg) with no iterations: dL/dC = -G plan / reg.
     n, reg = 64, 0.8
