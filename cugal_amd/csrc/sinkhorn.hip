@@ -399,10 +399,12 @@ constexpr int kCombWaves = 4;
 // The error bookkeeping shared by the combines: wave 0 of each block holds
 // the per-column error terms; the blocks' FP64 sums are totalled in fixed
 // order by the last block to arrive, which records the error, raises `done`
-// below the threshold and otherwise updates the Lehmann factor.
+// below the threshold and otherwise updates the Lehmann factor.  It also
+// stores the stop decision into `host_flag` (pinned host memory, written at
+// system scope) when given, so the host polls a flag instead of copying one.
 __device__ void combine_error(double err, int want_err, int check_index, int cur_buf, double threshold,
                               int eval_freq, int momentum_start, double* __restrict__ errpart,
-                              SinkCtl* __restrict__ ctl) {
+                              SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
     __shared__ double dred[kCombWaves];
     __shared__ int is_last;
     const int lane = threadIdx.x & (kCombCols - 1);
@@ -442,6 +444,7 @@ __device__ void combine_error(double err, int want_err, int check_index, int cur
         } else {
             ctl->momentum = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
         }
+        if (host_flag) __hip_atomic_store(host_flag, sum < threshold ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -449,7 +452,7 @@ template <bool PAIRS>
 __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
     const void* __restrict__ part_raw, int nblk, int n, const float* __restrict__ v2_in, float* __restrict__ v2_out,
     int raw_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
-    double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
+    double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
     __shared__ float rm[kCombWaves][kCombCols];
     __shared__ double rs[kCombWaves][kCombCols];
     if (ctl->done) return;
@@ -500,7 +503,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         v2_out[j] = static_cast<float>(raw_out ? (L - vj) : (vj - L));
         if (want_err) err = fabs(exp2(L) - 1.0);
     }
-    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl);
+    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
 }
 
 // v_next = (1 - m) v - m * lse  (only when Lehmann momentum is configured).
@@ -681,7 +684,7 @@ __global__ __launch_bounds__(W * kWave) void std_row_sweep(const float* __restri
 __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
     const float* __restrict__ part, int nblk, int n, const float* __restrict__ v_in, float* __restrict__ v_out,
     int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
-    double* __restrict__ errpart, SinkCtl* __restrict__ ctl) {
+    double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
     __shared__ double rs[kCombWaves][kCombCols];
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
@@ -701,7 +704,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
         v_out[j] = static_cast<float>(1.0 / (S + static_cast<double>(kStdEps)));
         if (want_err) err = fabs(static_cast<double>(v_in[j]) * S - 1.0);
     }
-    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl);
+    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
 }
 
 // v_next = v^(1-m) * raw^m once the check of this combine has set m.
@@ -1077,10 +1080,10 @@ int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, cons
     auto comb = [&](bool pairs, const float* vin, float* vout) -> int {
         if (pairs)
             hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.col_nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl);
+                               p.col_nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
         else
             hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl);
+                               p.nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
         return check_launch("sk_combine");
     };
     if (steps <= 0) return CG_OK;
@@ -1168,11 +1171,19 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     std::vector<Pending> pending;
     int next_slot = 0;
     bool stopped = false;
+    // A checking combine stores its stop decision into a cleared pinned slot;
+    // the host reads the slot once the event after the combine has passed.
+    int check_slot = 0;
+    auto check_flag = [&](int want_err) -> int* {
+        if (!want_err) return nullptr;
+        check_slot = next_slot++ & 63;
+        g_done_slots[check_slot] = 0;
+        return g_done_slots + check_slot;
+    };
     auto post_check = [&]() -> int {
         hipEvent_t ev;
         CG_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        const int slot = next_slot++ & 63;
-        CG_CHECK_HIP(hipMemcpyAsync(g_done_slots + slot, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        const int slot = check_slot;
         CG_CHECK_HIP(hipEventRecord(ev, s));
         pending.push_back({ev, slot});
         if (pending.size() >= 2) {
@@ -1186,14 +1197,15 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     };
     // pairs: partials of the exact column sweep; else sums of the fused sweep.
     auto combine = [&](bool pairs, int cur, int want_err, int check_index) -> int {
+        int* flag = check_flag(want_err);
         if (pairs)
             hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
                                p.col_nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
-                               threshold, eval_freq, momentum_start, w.errpart, w.ctl);
+                               threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         else
             hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
                                p.nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
-                               threshold, eval_freq, momentum_start, w.errpart, w.ctl);
+                               threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         if (int rc = check_launch("sk_combine")) return rc;
         if (momentum) {
             hipLaunchKernelGGL(sk_apply_momentum, vgrid, dim3(256), 0, s, w.v2[cur], w.v2[cur ^ 1], ni, w.ctl);
@@ -1311,11 +1323,19 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
     std::vector<Pending> pending;
     int next_slot = 0;
     bool stopped = false;
+    // A checking combine stores its stop decision into a cleared pinned slot;
+    // the host reads the slot once the event after the combine has passed.
+    int check_slot = 0;
+    auto check_flag = [&](int want_err) -> int* {
+        if (!want_err) return nullptr;
+        check_slot = next_slot++ & 63;
+        g_done_slots[check_slot] = 0;
+        return g_done_slots + check_slot;
+    };
     auto post_check = [&]() -> int {
         hipEvent_t ev;
         CG_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        const int slot = next_slot++ & 63;
-        CG_CHECK_HIP(hipMemcpyAsync(g_done_slots + slot, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        const int slot = check_slot;
         CG_CHECK_HIP(hipEventRecord(ev, s));
         pending.push_back({ev, slot});
         if (pending.size() >= 2) {
@@ -1328,9 +1348,10 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
         return 0;
     };
     auto combine = [&](int nblk, int cur, int want_err, int check_index) -> int {
+        int* flag = check_flag(want_err);
         hipLaunchKernelGGL(std_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, part, nblk, ni,
                            w.v2[cur], w.v2[cur ^ 1], want_err, check_index, cur, threshold, eval_freq, momentum_start,
-                           w.errpart, w.ctl);
+                           w.errpart, w.ctl, flag);
         if (int rc = check_launch("std_combine")) return rc;
         if (momentum) {
             hipLaunchKernelGGL(std_apply_momentum, vgrid, dim3(256), 0, s, w.v2[cur], w.v2[cur ^ 1], ni, w.ctl);
