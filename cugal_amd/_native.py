@@ -80,6 +80,8 @@ _SIGNATURES = {
     "cg_lap_greedy_random": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "cg_lap_hungarian_workspace": (c_sz, [c_i64]),
     "cg_lap_hungarian": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_p, c_sz, c_p]),
+    "cg_alignment_counts_workspace": (c_sz, [c_i64, c_i64]),
+    "cg_alignment_counts": (c_int, [c_p, c_p, c_i64, c_p, c_p, c_i64, c_p, c_p, c_i64, c_p, c_p, c_sz, c_p]),
 }
 
 

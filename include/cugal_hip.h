@@ -309,6 +309,28 @@ size_t cg_lap_hungarian_workspace(int64_t n);
 int cg_lap_hungarian(const float* cost, int64_t n, int maximize, int32_t* assignment, int32_t* stats,
                      void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Alignment quality (benchmark harness).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces the dense numpy metrics edge_correctness,
+ * induced_conserved_structure and symmetric_substructure
+ * (benchmark/experiment.py:115-141, called from Result.calculate, :365-376).
+ * S (n_source rows) and T (n_target rows) are CSRs in the graphs' node order
+ * (the row order nx.to_numpy_array gives them); answer[m] is the computed
+ * alignment and truth[m] the ground-truth permutation (device int32).
+ * counts (device, 4 int64) receives
+ *   [0] inter  = #{(a, b): S[answer[a]][answer[b]] = 1 and T[truth[a]][truth[b]] = 1},
+ *   [1] ones_s = #{S == 1},  [2] ones_t = #{T[truth][:, truth] == 1},
+ *   [3] flags: bit 0 an answer outside [-n_source, n_source) (the reference
+ *       then scores 0), bit 1 truth is not a permutation of distinct nodes;
+ * negative answers index from the end, as numpy does.  EC = inter / ones_s,
+ * ICS = inter / ones_t, S3 = inter / (ones_s + ones_t - inter). */
+size_t cg_alignment_counts_workspace(int64_t n_target, int64_t m);
+int cg_alignment_counts(const int32_t* s_col, const int32_t* s_rowptr, int64_t n_source, const int32_t* t_col,
+                        const int32_t* t_rowptr, int64_t n_target, const int32_t* answer, const int32_t* truth,
+                        int64_t m, int64_t* counts, void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

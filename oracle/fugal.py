@@ -407,3 +407,48 @@ def unrolled_plan(C: torch.Tensor, reg: float, iterations: int, u0=None, v0=None
 def feature_cost(fs: torch.Tensor, ft: torch.Tensor, w: torch.Tensor, lam) -> torch.Tensor:
     """lam * sum_k w_k |fs_ik - ft_jk| (weighted L1 over the features)."""
     return lam * (w[None, None, :] * (fs[:, None, :] - ft[None, :, :]).abs()).sum(-1)
+
+
+# ---------------------------------------------------------------------------
+# Alignment quality (benchmark/experiment.py:115-141)
+# ---------------------------------------------------------------------------
+
+def _aligned_adjacencies(source_adjacency, target_adjacency, source_mapping, target_mapping):
+    adj1 = source_adjacency[source_mapping][:, source_mapping]
+    adj2 = target_adjacency[target_mapping][:, target_mapping]
+    return adj1, adj2
+
+
+def edge_correctness(source_adjacency, target_adjacency, source_mapping, target_mapping) -> float:
+    """benchmark/experiment.py:115-121, dense numpy (0 on any error, as there).
+    Called as Result.calculate does (:365-376): source_mapping = the computed
+    answer, target_mapping = the ground truth, adjacencies from
+    nx.to_numpy_array (rows in the graphs' node order).
+    Pin: brute force over ordered node pairs on small graphs, 1 for the
+    ground-truth answer on a noise-free pair, and the identities
+    EC * #S = ICS * #T = conserved pairs."""
+    try:
+        adj1, adj2 = _aligned_adjacencies(source_adjacency, target_adjacency, source_mapping, target_mapping)
+        return np.sum(adj1 + adj2 == 2) / np.sum(source_adjacency == 1)
+    except Exception:
+        return 0
+
+
+def induced_conserved_structure(source_adjacency, target_adjacency, source_mapping, target_mapping) -> float:
+    """benchmark/experiment.py:124-130.  Pin: as edge_correctness."""
+    try:
+        adj1, adj2 = _aligned_adjacencies(source_adjacency, target_adjacency, source_mapping, target_mapping)
+        return np.sum(adj1 + adj2 == 2) / np.sum(adj2 == 1)
+    except Exception:
+        return 0
+
+
+def symmetric_substructure(source_adjacency, target_adjacency, source_mapping, target_mapping) -> float:
+    """benchmark/experiment.py:133-140.  Pin: as edge_correctness, and
+    S3 = inter / (#S + #T - inter) from the brute-force counts."""
+    try:
+        adj1, adj2 = _aligned_adjacencies(source_adjacency, target_adjacency, source_mapping, target_mapping)
+        intersection = np.sum(adj1 + adj2 == 2)
+        return intersection / (np.sum(source_adjacency == 1) + np.sum(adj2 == 1) - intersection)
+    except Exception:
+        return 0

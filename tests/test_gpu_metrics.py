@@ -1,0 +1,119 @@
+"""Structure scores on the device (cg_alignment_counts) against the dense numpy
+restatement of benchmark/experiment.py:115-141, and the experiment harness
+end to end."""
+
+import json
+
+import networkx as nx
+import numpy as np
+import pytest
+import torch
+
+from cugal_amd import Config, HungarianMethod, SinkhornMethod
+from cugal_amd.experiment import (Algorithm, Experiment, ExperimentResults, Graph, GraphKind, NoiseLevel,
+                                  generate_graph, structure_scores)
+from oracle import fugal as oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_scores(source, target, answer, mapping):
+    S = nx.to_numpy_array(source) if isinstance(source, nx.Graph) else source
+    T = nx.to_numpy_array(target) if isinstance(target, nx.Graph) else target
+    return (oracle.induced_conserved_structure(S, T, answer, mapping), oracle.edge_correctness(S, T, answer, mapping),
+            oracle.symmetric_substructure(S, T, answer, mapping))
+
+
+def _instance(n, p, noise, seed):
+    # Graphs as the harness builds them: from edge lists, so their node order
+    # is the order of first appearance, not the labels' order.
+    gen = np.random.default_rng(seed)
+    g = nx.erdos_renyi_graph(n, p, seed=seed)
+    source, target, mapping, _ = generate_graph(g, None, gen, NoiseLevel(0.0, noise, True))
+    m = len(mapping)
+    source.add_nodes_from(range(m))
+    target.add_nodes_from(range(m))
+    return source, target, mapping
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_scores_match_oracle(device, seed):
+    source, target, mapping = _instance(300, 0.03, 0.05, seed)
+    assert list(target.nodes()) != sorted(target.nodes())
+    rng = np.random.default_rng(seed)
+    m = len(mapping)
+    answers = {
+        "random": rng.permutation(m),
+        "truth": np.asarray(mapping),
+        "identity": np.arange(m),
+        "duplicates": np.where(rng.random(m) < 0.2, 0, rng.permutation(m)),
+        "negative": np.where(rng.random(m) < 0.1, -1 - rng.integers(0, m, m), np.arange(m)),
+    }
+    for name, ans in answers.items():
+        got = structure_scores(source, target, ans, mapping, device)
+        want = _oracle_scores(source, target, ans, mapping)
+        assert got == tuple(float(x) for x in want), (name, got, want)
+
+
+def test_scores_errors_and_special_graphs(device):
+    source, target, mapping = _instance(120, 0.05, 0.0, 4)
+    m = len(mapping)
+    # Out-of-range answers and length mismatches score 0, as the reference's
+    # try/except does.
+    bad = np.arange(m)
+    bad[3] = m
+    assert structure_scores(source, target, bad, mapping, device) == (0.0, 0.0, 0.0)
+    assert _oracle_scores(source, target, bad, mapping) == (0, 0, 0)
+    assert structure_scores(source, target, np.arange(m - 1), mapping, device) == (0.0, 0.0, 0.0)
+    # Self-loops are one entry of the dense matrix; a directed pair; dense input.
+    source.add_edge(5, 5)
+    target.add_edge(7, 7)
+    ans = np.random.default_rng(1).permutation(m)
+    assert structure_scores(source, target, ans, mapping, device) == _oracle_scores(source, target, ans, mapping)
+    ds = nx.DiGraph(nx.gnp_random_graph(60, 0.1, seed=3, directed=True))
+    dt = nx.DiGraph(nx.gnp_random_graph(60, 0.1, seed=4, directed=True))
+    perm = np.random.default_rng(2).permutation(60)
+    ans = np.random.default_rng(3).permutation(60)
+    assert structure_scores(ds, dt, ans, perm, device) == _oracle_scores(ds, dt, ans, perm)
+    S, T = nx.to_numpy_array(source), nx.to_numpy_array(target)
+    ans = np.random.default_rng(5).permutation(m)
+    assert structure_scores(S, T, ans, mapping, device) == _oracle_scores(S, T, ans, mapping)
+    with pytest.raises(ValueError):
+        structure_scores(source, target, ans, np.zeros(m, dtype=np.int64), device)
+
+
+def test_standard_scores_of_the_true_alignment(device):
+    source, target, mapping = _instance(200, 0.04, 0.0, 6)
+    # Noise free: the true alignment conserves every edge.
+    assert structure_scores(source, target, mapping, mapping, device, standard=True) == (1.0, 1.0, 1.0)
+    assert structure_scores(source, target, np.arange(len(mapping)), mapping, device, standard=False) != (1.0,) * 3
+
+
+def test_experiment_runs_end_to_end(device, tmp_path):
+    # Two predefined edge-list files and a Newman-Watts family, two noise
+    # levels, two algorithms; results written and read back as JSON.
+    g = nx.watts_strogatz_graph(120, 6, 0.1, seed=1)
+    src_file, tgt_file = tmp_path / "s.txt", tmp_path / "t.txt"
+    src_file.write_text("# source\n" + "\n".join(f"{u} {v}" for u, v in g.edges()) + "\n")
+    # The reference takes a predefined pair as aligned by the identity and
+    # permutes the target itself (benchmark/experiment.py:313-324).
+    tgt_file.write_text("\n".join(f"{u}\t{v}" for u, v in g.edges()) + "\n")
+    base = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                  sinkhorn_regularization=0.3, mu=2.0, iter_count=4, frank_wolfe_iter_count=4,
+                  use_sparse_adjacency=True)
+    algos = [Algorithm(base, False),
+             Algorithm(Config(**{**base.__dict__, "hungarian_method": HungarianMethod.SCIPY}), False)]
+    exp = Experiment(algorithms=algos,
+                     graphs=[Graph(GraphKind.NEWMAN_WATTS, {"n": 150, "k": 6, "p": 0.1}),
+                             Graph(GraphKind.PREDEFINED_GRAPHS, {"source_file": str(src_file),
+                                                                 "target_file": str(tgt_file)})],
+                     noise_levels=[NoiseLevel(0.0, 0.0, True), NoiseLevel(0.0, 0.05, True)], seed=11, num_runs=1)
+    res = exp.run()
+    assert len(res.results) == 2 and all(len(r) == 2 and all(len(x) == 2 for x in r) for r in res.results)
+    for graph, level, algo, result in res.all_results():
+        for v in (result.ics, result.ec, result.sss, result.accuracy):
+            assert 0.0 <= v <= 1.0
+        assert result.profile.frank_wolfe_iterations == 16
+    res.dump(str(tmp_path))
+    back = ExperimentResults.from_dict(json.load(open(tmp_path / (res.name() + ".json"))))
+    assert back.to_dict() == res.to_dict()

@@ -155,3 +155,41 @@ def test_auto_gradient_precision_follows_density():
     assert gradient_precision_for(adj(1), adj(2 * int(threshold) + 4), "auto") == "bf16x2"
     for explicit in ("fp32", "bf16x2"):
         assert gradient_precision_for(dense, sparse, explicit) == explicit
+
+
+def test_snap_edge_list_parser():
+    from cugal_amd.experiment import create_graph_from_str
+
+    text = "# Directed graph: example\n# FromNodeId\tToNodeId\n0 1\n1\t2\n\n2 0\n3 4 5\n7 7\n"
+    g = create_graph_from_str(text)
+    assert sorted(g.edges()) == [(0, 1), (0, 2), (1, 2), (7, 7)]
+    assert list(g.nodes()) == [0, 1, 2, 7]
+
+
+def test_experiment_results_json_round_trip(tmp_path):
+    import json
+
+    from cugal_amd.experiment import (Algorithm, Experiment, ExperimentResults, Graph, GraphKind, NoiseLevel,
+                                      Result)
+    from cugal_amd.profile import Phase, Profile, SinkhornProfile
+
+    exp = Experiment(algorithms=[Algorithm(Config(device="cuda", dtype=torch.float32), False)],
+                     graphs=[Graph(GraphKind.NEWMAN_WATTS, {"n": 30, "k": 4, "p": 0.1}),
+                             Graph(GraphKind.CA_NETSCIENCE, {})],
+                     noise_levels=[NoiseLevel(0.0, 0.05, True)], seed=3, num_runs=2)
+    prof = Profile([SinkhornProfile([0.1], 10, 0.5)], {Phase.SINKHORN: 0.5}, 1.0, None, 4, 0.0)
+    res = ExperimentResults(exp, "abc", "2026-01-01 00:00:00",
+                            [[[Result(0.5, 0.25, 0.2, 0.9, prof)]], [[Result(1.0, 1.0, 1.0, 1.0, prof)]]])
+    res.dump(str(tmp_path))
+    path = tmp_path / (res.name() + ".json")
+    data = json.load(open(path))
+    # The keys of benchmark/experiment.py:515-560 and :382-392.
+    assert set(data) == {"experiment", "commit", "time", "results"}
+    assert set(data["experiment"]) == {"algorithms", "graphs", "noise_levels", "debug", "seed", "save_alignment",
+                                       "num_runs"}
+    assert set(data["results"][0][0][0]) == {"ics", "ec", "sss", "accuracy", "profile"}
+    assert data["experiment"]["graphs"][1] == {"kind": "CA_NETSCIENCE", "parameters": {}}
+    back = ExperimentResults.from_dict(data)
+    assert back.to_dict() == res.to_dict()
+    assert [str(g) for g, _, _, _ in back.all_results()] == [str(g) for g in exp.graphs]
+    assert res.name().startswith("NEWMAN_WATTS-CA_NETSCIENCE-")

@@ -366,3 +366,62 @@ def test_unrolled_plan_is_solve_log_and_feature_cost_brute_force():
     want = np.array([[0.3 * sum(float(w[k]) * abs(float(fs[i, k] - ft[j, k])) for k in range(4)) for j in range(5)]
                      for i in range(5)])
     np.testing.assert_allclose(oracle.feature_cost(fs, ft, w, 0.3).numpy(), want, rtol=1e-14)
+
+
+# ---- alignment quality (benchmark/experiment.py:115-141) ----------------------
+
+def _brute_counts(S, T, ans, gt):
+    """Conserved ordered node pairs by explicit enumeration."""
+    m = len(ans)
+    inter = sum(1 for a in range(m) for b in range(m) if S[ans[a], ans[b]] == 1 and T[gt[a], gt[b]] == 1)
+    ones_t = sum(1 for a in range(m) for b in range(m) if T[gt[a], gt[b]] == 1)
+    return inter, int((S == 1).sum()), ones_t
+
+
+def _scores(S, T, ans, gt):
+    return (oracle.induced_conserved_structure(S, T, ans, gt), oracle.edge_correctness(S, T, ans, gt),
+            oracle.symmetric_substructure(S, T, ans, gt))
+
+
+def test_structure_scores_hand_cases():
+    # S: path 0-1-2; T: star around 0 (edges 0-1, 0-2); ground truth identity.
+    S = nx.to_numpy_array(nx.Graph([(0, 1), (1, 2)]), nodelist=[0, 1, 2])
+    T = nx.to_numpy_array(nx.Graph([(0, 1), (0, 2)]), nodelist=[0, 1, 2])
+    gt = np.array([0, 1, 2])
+    # Answer (1, 0, 2) maps S's edges onto T's: everything is conserved.
+    assert _scores(S, T, np.array([1, 0, 2]), gt) == (1.0, 1.0, 1.0)
+    # The identity answer conserves one of two edges: 2 of 4 ordered pairs.
+    ics, ec, s3 = _scores(S, T, gt, gt)
+    assert (ics, ec) == (0.5, 0.5) and abs(s3 - 1 / 3) < 1e-15
+    # An answer outside the graph scores 0 (the reference catches the error).
+    assert _scores(S, T, np.array([0, 1, 3]), gt) == (0, 0, 0)
+
+
+def test_structure_scores_brute_force_and_noise_free_pairs():
+    from cugal_amd.synthetic import erdos_renyi
+
+    rng = np.random.default_rng(3)
+    for seed in range(3):
+        inst = erdos_renyi(40, 0.15, 0.1, seed=seed)
+        # cugal() pads both graphs before the harness scores them (isolated
+        # nodes are missing from an edge list).
+        inst.source.add_nodes_from(range(40))
+        inst.target.add_nodes_from(range(40))
+        S = nx.to_numpy_array(inst.source)
+        T = nx.to_numpy_array(inst.target)
+        gt = inst.source_mapping
+        ans = rng.permutation(len(gt))
+        inter, ones_s, ones_t = _brute_counts(S, T, ans, gt)
+        ics, ec, s3 = _scores(S, T, ans, gt)
+        assert ec == inter / ones_s and ics == inter / ones_t and s3 == inter / (ones_s + ones_t - inter)
+    # A noise-free pair, graphs in sorted node order.  The reference applies
+    # the answer to S and the truth to T, so T[truth][:, truth] = S and the
+    # identity answer scores 1, while the true alignment in general does not
+    # (the standard definition, S against T[answer][:, answer], is this
+    # formula with (identity, answer), and scores the truth 1).
+    inst = erdos_renyi(50, 0.1, 0.0, seed=7)
+    n = len(inst.source_mapping)
+    S = nx.to_numpy_array(inst.source, nodelist=range(n))
+    T = nx.to_numpy_array(inst.target, nodelist=range(n))
+    assert _scores(S, T, np.arange(n), inst.source_mapping) == (1.0, 1.0, 1.0)
+    assert _scores(S, T, inst.source_mapping, inst.source_mapping)[1] < 0.5
