@@ -41,15 +41,17 @@ _SIGNATURES = {
     "cg_node_features_workspace": (c_sz, [c_i64]),
     "cg_node_features": (c_int, [c_p, c_p, c_i64, c_f, c_p, c_p, c_p, c_sz, c_p]),
     "cg_egonet_edges": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "cg_fugal_features_workspace": (c_sz, [c_i64]),
+    "cg_fugal_features": (c_int, [c_p, c_p, c_i64, c_f, c_p, c_p, c_sz, c_p]),
     "cg_add_distance": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_regularize": (c_int, [c_p, c_p, c_i64, c_int, c_p]),
     "cg_gradient_workspace": (c_sz, [c_i64]),
     "cg_gradient_padded_ld": (c_i64, [c_i64]),
-    "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_int, c_f, c_p, c_p, c_sz, c_p]),
     "cg_adjacency_bits_words": (c_sz, [c_i64]),
     "cg_adjacency_pack_bits": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_gradient_mfma_workspace": (c_sz, [c_i64]),
-    "cg_gradient_mfma": (c_int, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_gradient_mfma": (c_int, [c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_int, c_f, c_p, c_p, c_sz, c_p]),
     "cg_sinkhorn_log_step": (c_int, [c_p, c_i64, c_p, c_p, c_f, c_p]),
     "cg_sinkhorn_log_marginal_workspace": (c_sz, [c_i64]),
     "cg_sinkhorn_log_marginal": (c_int, [c_p, c_i64, c_p, c_p, c_p, c_p, c_sz, c_p]),

@@ -127,6 +127,12 @@ class Config:
     matrix cores when the adjacencies hold more than ~0.77 % of n^2
     non-zeros, where they become the faster form, else "fp32")."""
 
+    node_features: str = "cugal"
+    """The node features behind the distance term: "cugal" (the 4 of
+    cugal/feature_extraction.py:41-64) or "fugal7" (the 7 of the original
+    FUGAL, FUGAL/pred.py:24-110 with simple=False: the same 4 plus ego-net
+    edges, edges leaving the ego net and nodes at distance 2)."""
+
     def convert_tensor(self, input: torch.Tensor) -> torch.Tensor:
         """Convert tensor to correct type and dtype."""
         return input.to(dtype=self.dtype, device=self.device)

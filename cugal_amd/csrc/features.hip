@@ -197,6 +197,93 @@ __global__ void egonet_kernel(const int32_t* __restrict__ rp, const int32_t* __r
     out[v] = (rp[v + 1] - rp[v]) + tri[v] / 2;
 }
 
+
+// The three extra node features of FUGAL's 7-feature set (FUGAL/pred.py:
+// 24-110 with simple=False) for a simple graph, next to the 4 of
+// vertex_features4 (already scaled by mu), as 8-float rows
+// (deg, clustering, mean neighbour degree, mean neighbour clustering,
+//  ego-net edges, edges leaving the ego net, distinct nodes at distance 2, 0):
+//   ego(v)  = deg(v) + t(v)                  (t = triangles at v),
+//   out(v)  = sum_{u in N(v)} deg(u) - deg(v) - 2 t(v)
+//             (each ego-net edge is counted twice by the degree sum of the
+//              closed neighbourhood, each leaving edge once),
+//   hop2(v) = |N(N(v)) \ N[v]|              (0 for an isolated vertex).
+// One block per vertex (grid-stride); two bitmaps of n bits in LDS hold the
+// closed neighbourhood and the distance-2 set, which is counted at first
+// insertion; both are cleared by revisiting what was set, so a vertex costs
+// O(sum of its neighbours' degrees).
+constexpr int kFugalThreads = 256;
+
+__global__ __launch_bounds__(kFugalThreads) void fugal_extra(const int32_t* __restrict__ col,
+                                                            const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ tri,
+                                                            const float* __restrict__ f4, int n, float mu,
+                                                            float* __restrict__ out8) {
+    extern __shared__ uint32_t bits[];
+    __shared__ int wsum[2][kFugalThreads / kWave];
+    const int words = (n + 31) >> 5;
+    uint32_t* nb = bits;          // closed neighbourhood
+    uint32_t* hb = bits + words;  // distance-2 set
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    for (int w = tid; w < 2 * words; w += kFugalThreads) bits[w] = 0u;
+    __syncthreads();
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+        const int b = rp[v];
+        const int e = rp[v + 1];
+        int s1 = 0;
+        for (int k = b + tid; k < e; k += kFugalThreads) {
+            const int u = col[k];
+            atomicOr(&nb[u >> 5], 1u << (u & 31));
+            s1 += rp[u + 1] - rp[u];
+        }
+        if (tid == 0) atomicOr(&nb[v >> 5], 1u << (v & 31));
+        __syncthreads();
+        int hop = 0;
+        for (int k = b; k < e; ++k) {
+            const int u = col[k];
+            const int ue = rp[u + 1];
+            for (int q = rp[u] + tid; q < ue; q += kFugalThreads) {
+                const int x = col[q];
+                const uint32_t bit = 1u << (x & 31);
+                if (!(nb[x >> 5] & bit) && !(atomicOr(&hb[x >> 5], bit) & bit)) ++hop;
+            }
+        }
+        s1 = wave_sum(s1);
+        hop = wave_sum(hop);
+        if (lane == 0) {
+            wsum[0][wave] = s1;
+            wsum[1][wave] = hop;
+        }
+        __syncthreads();  // every read of nb and insertion into hb is done
+        for (int k = b; k < e; ++k) {
+            const int u = col[k];
+            const int ue = rp[u + 1];
+            for (int q = rp[u] + tid; q < ue; q += kFugalThreads) hb[col[q] >> 5] = 0u;
+        }
+        for (int k = b + tid; k < e; k += kFugalThreads) nb[col[k] >> 5] = 0u;
+        if (tid == 0) {
+            nb[v >> 5] = 0u;
+            int S1 = 0, H = 0;
+            for (int w = 0; w < kFugalThreads / kWave; ++w) {
+                S1 += wsum[0][w];
+                H += wsum[1][w];
+            }
+            const int d = e - b;
+            const int t = tri[v] / 2;
+            const float4 a = *reinterpret_cast<const float4*>(f4 + 4 * static_cast<int64_t>(v));
+            float* o = out8 + 8 * static_cast<int64_t>(v);
+            *reinterpret_cast<float4*>(o) = a;
+            *reinterpret_cast<float4*>(o + 4) =
+                d == 0 ? make_float4(0.f, 0.f, 0.f, 0.f)
+                       : make_float4(mu * static_cast<float>(d + t), mu * static_cast<float>(S1 - d - 2 * t),
+                                     mu * static_cast<float>(H), 0.0f);
+        }
+        __syncthreads();  // bitmaps clear and wsum read before the next vertex
+    }
+}
+
 bool bad_n(int64_t n) { return n <= 0 || n >= (int64_t(1) << 30); }
 
 }  // namespace
@@ -304,6 +391,46 @@ int cg_egonet_edges(const int32_t* row_pointers, const int32_t* triangles, int64
     hipLaunchKernelGGL(egonet_kernel, dim3(static_cast<unsigned>(div_up(n, 256))), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), row_pointers, triangles, static_cast<int>(n), out);
     return check_launch("egonet_kernel");
+}
+
+size_t cg_fugal_features_workspace(int64_t n) {
+    if (n <= 0) return 0;
+    Carver cv{nullptr, 0, ~size_t(0)};
+    cv.take<int32_t>(static_cast<size_t>(n));
+    cv.take<float>(4 * static_cast<size_t>(n));
+    return cv.used + 256;
+}
+
+int cg_fugal_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, float mu, float* out,
+                      void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    if (!row_pointers || !out || bad_n(n)) return fail(CG_EINVAL, "cg_fugal_features: bad arguments");
+    if (!aligned16(out)) return fail(CG_EINVAL, "cg_fugal_features: out must be 16-byte aligned");
+    if (!workspace || workspace_bytes < cg_fugal_features_workspace(n))
+        return fail(CG_EWORKSPACE, "cg_fugal_features: workspace too small");
+    const size_t lds = sizeof(uint32_t) * 2 * static_cast<size_t>(div_up(n, 32));
+    if (lds > 160 * 1024) return fail(CG_EUNSUPPORTED, "cg_fugal_features: n = %lld exceeds the LDS bitmaps",
+                                      static_cast<long long>(n));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    int32_t* tri = cv.take<int32_t>(static_cast<size_t>(n));
+    float* f4 = cv.take<float>(4 * static_cast<size_t>(n));
+    const int ni = static_cast<int>(n);
+    const dim3 grid(static_cast<unsigned>(div_up(n, 4)));
+    hipLaunchKernelGGL(vertex_triangles, grid, dim3(256), 0, s, col_indices, row_pointers, ni, tri,
+                       static_cast<float*>(nullptr), static_cast<float*>(nullptr));
+    if (int rc = check_launch("vertex_triangles")) return rc;
+    hipLaunchKernelGGL(vertex_features4, grid, dim3(256), 0, s, col_indices, row_pointers, tri, ni, mu, f4);
+    if (int rc = check_launch("vertex_features4")) return rc;
+    static size_t attr = 0;
+    if (lds > attr) {
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fugal_extra),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        attr = lds;
+    }
+    const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(n, 2048));
+    hipLaunchKernelGGL(fugal_extra, dim3(blocks), dim3(kFugalThreads), lds, s, col_indices, row_pointers, tri, f4, ni,
+                       mu, out);
+    return check_launch("fugal_extra");
 }
 
 }  // extern "C"

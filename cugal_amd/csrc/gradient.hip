@@ -28,9 +28,10 @@ constexpr int kNumXcd = 8;
 struct Epilogue {
     const float* P;   // nullptr: no regulariser term
     int64_t ldp;      // row stride of P
-    const float* fs;  // source features n x 4 (nullptr: no distance term)
-    const float* ft;  // target features n x 4
+    const float* fs;  // source features n x fdim (nullptr: no distance term)
+    const float* ft;  // target features n x fdim
     float rl;         // f * lambda
+    int fdim;         // 4 or 8
 };
 
 __device__ __forceinline__ float feat_dist(float4 a, const float* __restrict__ ft, int j) {
@@ -39,9 +40,10 @@ __device__ __forceinline__ float feat_dist(float4 a, const float* __restrict__ f
     return sqrtf(fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3))));
 }
 
-__device__ __forceinline__ float epi_one(float acc, float scale, const Epilogue& ep, int64_t idx, float4 fsi, int j) {
+__device__ __forceinline__ float epi_one(float acc, float scale, const Epilogue& ep, int64_t idx, const FeatureRow& fsi,
+                                         int j) {
     float g = acc * scale;
-    if (ep.fs) g += feat_dist(fsi, ep.ft, j);
+    if (ep.fs) g += feature_distance(fsi, ep.ft, ep.fdim, j);
     if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[idx];
     return g;
 }
@@ -64,8 +66,8 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
     const int c0 = static_cast<int>(t / n) * chunk;
     const int b = arp[i];
     const int e = arp[i + 1];
-    float4 fsi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ep.fs) fsi = *reinterpret_cast<const float4*>(ep.fs + 4 * i);
+    FeatureRow fsi{};
+    if (ep.fs) fsi = feature_row(ep.fs, ep.fdim, i);
     if (VEC) {
         const int j = c0 + threadIdx.x * 4;
         if (j >= n) return;
@@ -235,8 +237,8 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
         const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
-            const float4 fsi = ep.fs ? *reinterpret_cast<const float4*>(ep.fs + 4 * static_cast<int64_t>(c))
-                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+            FeatureRow fsi{};
+            if (ep.fs) fsi = feature_row(ep.fs, ep.fdim, c);
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 if (rq + e < n) v[e] = epi_one(v[e], scale, ep, pbase + rq + e, fsi, rq + e);
@@ -340,7 +342,7 @@ int cg_adjacency_matmul(const int32_t* col_indices, const int32_t* row_pointers,
                         float* out, int negate, cg_stream_t stream) {
     if (!col_indices || !row_pointers || !matrix || !out || bad_n(n))
         return fail(CG_EINVAL, "cg_adjacency_matmul: bad arguments");
-    Epilogue ep{nullptr, 0, nullptr, nullptr, 0.0f};
+    Epilogue ep{nullptr, 0, nullptr, nullptr, 0.0f, 4};
     return launch_rowgather(col_indices, row_pointers, static_cast<int>(n), matrix, negate ? -1.0f : 1.0f, ep, out,
                             reinterpret_cast<hipStream_t>(stream));
 }
@@ -382,14 +384,15 @@ int64_t cg_gradient_padded_ld(int64_t n) { return n > 0 ? gradient_ld(n) : 0; }
 
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
                 int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
-                int lambda_, float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
-                cg_stream_t stream) {
+                int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
+                size_t workspace_bytes, cg_stream_t stream) {
     if (!a_col || !a_rowptr || !b_col || !b_rowptr || !P || !gradient || bad_n(n) || ld_p < n)
         return fail(CG_EINVAL, "cg_gradient: bad arguments");
     if ((source_features == nullptr) != (target_features == nullptr))
         return fail(CG_EINVAL, "cg_gradient: give both feature matrices or neither");
     if (source_features && (!aligned16(source_features) || !aligned16(target_features)))
         return fail(CG_EINVAL, "cg_gradient: feature matrices must be 16-byte aligned");
+    if (feature_dim != 4 && feature_dim != 8) return fail(CG_EINVAL, "cg_gradient: feature_dim must be 4 or 8");
     if (!workspace || workspace_bytes < cg_gradient_workspace(n))
         return fail(CG_EWORKSPACE, "cg_gradient: workspace too small");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -398,7 +401,7 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
     float* Pp = cv.take<float>(static_cast<size_t>(n) * ldy);
     float* Y = cv.take<float>(static_cast<size_t>(n) * ldy);
-    const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f};
+    const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f, 4};
     const float* X1 = P;
     int64_t ld1 = ld_p;
     if (n % 4 == 0 && aligned16(P) && !gather_friendly_ld(ld_p)) {
@@ -410,7 +413,7 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     }
     if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
-    Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl};
+    Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim};
     return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
 }
 

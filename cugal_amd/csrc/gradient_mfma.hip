@@ -119,12 +119,13 @@ __global__ __launch_bounds__(256) void split_transpose(const float* __restrict__
 struct GradEpi {
     const float* P;   // nullptr: no regulariser term
     int64_t ldp;      // row stride of P
-    const float* fs;  // source features n x 4 (nullptr: no distance term)
+    const float* fs;  // source features n x fdim (nullptr: no distance term)
     const float* ft;
     float rl;         // f * lambda
     float* G;         // n x n
     uint16_t* y_hi;   // gemm 1 output (np x np)
     uint16_t* y_lo;
+    int fdim;         // 4 or 8
 };
 
 // The grid has kNumXcd * ceil(tiles / kNumXcd) blocks; returns false for
@@ -308,8 +309,8 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int64_t m = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            float4 fsm = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (EPI == 1 && ep.fs && m < n) fsm = *reinterpret_cast<const float4*>(ep.fs + 4 * m);
+            FeatureRow fsm{};
+            if (EPI == 1 && ep.fs && m < n) fsm = feature_row(ep.fs, ep.fdim, m);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int64_t c = c0 + wn * 128 + j * 32 + (lane & 31);
@@ -321,11 +322,7 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
                     ep.y_lo[m * np + c] = l;
                 } else if (m < n && c < n) {
                     float g = y;
-                    if (ep.fs) {
-                        const float4 b = *reinterpret_cast<const float4*>(ep.ft + 4 * c);
-                        const float d0 = fsm.x - b.x, d1 = fsm.y - b.y, d2 = fsm.z - b.z, d3 = fsm.w - b.w;
-                        g += sqrtf(fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3))));
-                    }
+                    if (ep.fs) g += feature_distance(fsm, ep.ft, ep.fdim, c);
                     if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[m * ep.ldp + c];
                     ep.G[m * n + c] = g;
                 }
@@ -390,7 +387,7 @@ size_t cg_gradient_mfma_workspace(int64_t n) {
 }
 
 int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P, int64_t ld_p,
-                     const float* source_features, const float* target_features, int lambda_,
+                     const float* source_features, const float* target_features, int feature_dim, int lambda_,
                      float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                      cg_stream_t stream) {
     if (!a_bits || !b_bits || !P || !gradient || n <= 0 || n > (int64_t(1) << 20) || ld_p < n)
@@ -399,6 +396,7 @@ int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, 
         return fail(CG_EINVAL, "cg_gradient_mfma: give both feature matrices or neither");
     if (source_features && (!aligned16(source_features) || !aligned16(target_features)))
         return fail(CG_EINVAL, "cg_gradient_mfma: feature matrices must be 16-byte aligned");
+    if (feature_dim != 4 && feature_dim != 8) return fail(CG_EINVAL, "cg_gradient_mfma: feature_dim must be 4 or 8");
     if (!aligned16(a_bits) || !aligned16(b_bits))
         return fail(CG_EINVAL, "cg_gradient_mfma: packed adjacencies must be 16-byte aligned");
     if (!workspace || workspace_bytes < cg_gradient_mfma_workspace(n))
@@ -415,10 +413,11 @@ int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, 
     const unsigned tb = static_cast<unsigned>(np / 64);
     hipLaunchKernelGGL(split_transpose, dim3(tb, tb), dim3(256), 0, s, P, ld_p, ni, np, pt_hi, pt_lo);
     if (int rc = check_launch("split_transpose")) return rc;
-    GradEpi ep1{nullptr, 0, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo};
+    GradEpi ep1{nullptr, 0, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo, 4};
     if (int rc = launch_gemm<false, 0>(a_bits, pt_hi, pt_lo, ni, np, 1.0f, ep1, s, "gemm_nt<Y = A P>")) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
-    GradEpi ep2{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, gradient, nullptr, nullptr};
+    GradEpi ep2{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, gradient, nullptr, nullptr,
+                feature_dim};
     return launch_gemm<true, 1>(b_bits, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
 }
 

@@ -3,7 +3,9 @@ cugal/feature_extraction.py:15-108.
 
 Per node: degree, clustering coefficient, mean neighbour degree and mean
 neighbour clustering (the order of cugal/feature_extraction.py:64 and :38),
-computed by the HIP feature kernels from the CSR adjacency.  Integer counts
+computed by the HIP feature kernels from the CSR adjacency; with
+Config.node_features = "fugal7" also the three counts of the original
+FUGAL's 7-feature set (FUGAL/pred.py:24-110).  Integer counts
 are exact and every ratio is the FP32 rounding of the FP64 value networkx
 computes.
 """
@@ -35,6 +37,23 @@ def node_features(adjacency: Adjacency, mu: float = 1.0, triangles: torch.Tensor
     nat.call(
         "cg_node_features", nat.ptr(adjacency.col_indices), nat.ptr(adjacency.row_pointers), n, float(mu),
         nat.ptr(out), nat.ptr(triangles), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    return out
+
+
+def fugal_node_features(adjacency: Adjacency, mu: float = 1.0) -> torch.Tensor:
+    """n x 8 FP32 rows scaled by `mu`: the 7 features of the original FUGAL
+    (FUGAL/pred.py:24-110 with simple=False) -- degree, clustering, mean
+    neighbour degree, mean neighbour clustering, ego-net edges, edges leaving
+    the ego net, distinct nodes at distance 2 -- and a zero.  Simple graphs
+    (no self-loops); the counts are exact."""
+    n = adjacency.number_of_nodes()
+    dev = adjacency.row_pointers.device
+    out = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    ws = nat.workspace(dev, nat.query("cg_fugal_features_workspace", n), "features")
+    nat.call(
+        "cg_fugal_features", nat.ptr(adjacency.col_indices) if len(adjacency.col_indices) else None,
+        nat.ptr(adjacency.row_pointers), n, float(mu), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
     )
     return out
 
@@ -85,15 +104,25 @@ class Features:
             raise NotImplementedError("feature extraction runs on a ROCm device (config.device 'cuda')")
         if config.dtype != torch.float32:
             raise NotImplementedError("the feature kernels produce FP32 (config.dtype torch.float32)")
-        source_features = node_features(_as_adjacency(source, config), config.mu)
-        target_features = node_features(_as_adjacency(target, config), config.mu)
+        if config.node_features == "cugal":
+            extract = node_features
+        elif config.node_features == "fugal7":
+            extract = fugal_node_features
+        else:
+            raise ValueError(f"node_features must be 'cugal' or 'fugal7', got {config.node_features!r}")
+        source_features = extract(_as_adjacency(source, config), config.mu)
+        target_features = extract(_as_adjacency(target, config), config.mu)
         if config.safe_mode:
             assert source_features.isfinite().all(), "source feature tensor has NaN values"
             assert target_features.isfinite().all(), "target feature tensor has NaN values"
         return cls(source_features, target_features)
 
     def add_distance(self, out: torch.Tensor) -> torch.Tensor:
-        """out += Euclidean distance matrix of the feature rows."""
+        """out += Euclidean distance matrix of the feature rows (the 4-feature
+        rows of cuda_kernels/distance.cu; the gradient kernels recompute the
+        distance of either feature set themselves)."""
+        if self.source.shape[1] != 4:
+            raise NotImplementedError("add_distance takes the 4-feature rows of the reference kernel")
         nat.require_device_tensor(out, "out", torch.float32)
         n = out.shape[0]
         nat.call(

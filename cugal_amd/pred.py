@@ -77,21 +77,23 @@ def _gradient_into(
         P = P.contiguous()
     ldp = nat.row_stride(P, "P")
     fs = ft = None
+    fdim = 4
     if isinstance(features, Features):
         fs, ft = features.source.contiguous(), features.target.contiguous()
+        fdim = int(fs.shape[1])
     precision = gradient_precision_for(A, B, precision)
     if precision == "bf16x2":
         ws = nat.workspace(dev, nat.query("cg_gradient_mfma_workspace", n), "gradient_mfma")
         nat.call(
             "cg_gradient_mfma", nat.ptr(A.packed_bits()), nat.ptr(B.packed_bits()), n, nat.ptr(P), ldp, nat.ptr(fs),
-            nat.ptr(ft), int(iteration), float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+            nat.ptr(ft), fdim, int(iteration), float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
         )
     elif precision == "fp32":
         ws = nat.workspace(dev, nat.query("cg_gradient_workspace", n), "gradient")
         nat.call(
             "cg_gradient", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(B.col_indices),
-            nat.ptr(B.row_pointers), n, nat.ptr(P), ldp, nat.ptr(fs), nat.ptr(ft), int(iteration), float(factor),
-            nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+            nat.ptr(B.row_pointers), n, nat.ptr(P), ldp, nat.ptr(fs), nat.ptr(ft), fdim, int(iteration),
+            float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
         )
     else:
         raise NotImplementedError(f"gradient precision {precision!r}")
@@ -306,6 +308,8 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
     node_count = max(max(source.nodes()), max(target.nodes()))
     source.add_nodes_from(set(range(node_count)) - set(source.nodes()))
     target.add_nodes_from(set(range(node_count)) - set(target.nodes()))
+    if config.node_features == "fugal7" and (nx.number_of_selfloops(source) or nx.number_of_selfloops(target)):
+        raise ValueError("the 7-feature set is defined here for simple graphs (no self-loops)")
 
     source_adj = _adjacency_of(source, config)
     target_adj = _adjacency_of(target, config)

@@ -8,7 +8,9 @@
  *   - CSR adjacency: int32 col_indices[nnz], int32 row_pointers[n + 1], the
  *     columns of a row sorted ascending;
  *   - node-feature matrices are n x 4 FP32 row-major
- *     (degree, clustering, mean neighbour degree, mean neighbour clustering);
+ *     (degree, clustering, mean neighbour degree, mean neighbour clustering),
+ *     or n x 8 where an entry point takes a feature_dim (FUGAL's 7 features
+ *     and a zero, cg_fugal_features);
  *   - `stream` is a hipStream_t (NULL = the null stream); work is enqueued on
  *     it and nothing synchronises the device unless the entry point says so;
  *   - temporaries come from a caller-provided device workspace whose size the
@@ -95,6 +97,18 @@ int cg_node_features(const int32_t* col_indices, const int32_t* row_pointers, in
 int cg_egonet_edges(const int32_t* row_pointers, const int32_t* triangles, int64_t n, int32_t* out,
                     cg_stream_t stream);
 
+/* The 7-feature node set of the original FUGAL (feature_extraction with
+ * simple=False, FUGAL/pred.py:24-110) for a simple undirected graph, as
+ * 8-float rows scaled by mu (FUGAL/pred.py:134 scales the distance by mu,
+ * which is the same): degree, clustering, mean neighbour degree, mean
+ * neighbour clustering (as cg_node_features), ego-net edges
+ * (nx.ego_graph(G, v).number_of_edges()), edges leaving the ego net, distinct
+ * vertices at distance 2, and 0.  Isolated vertices get zeros.  out is n x 8
+ * FP32 (16-byte aligned); n is limited by two n-bit LDS bitmaps (n <= 655360). */
+size_t cg_fugal_features_workspace(int64_t n);
+int cg_fugal_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, float mu, float* out,
+                      void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Cost / gradient.
  * ---------------------------------------------------------------------- */
@@ -113,7 +127,8 @@ int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_
  * undirected graphs (A, B symmetric, given as CSR):
  *   gradient = -2 A P B + D + regularizer_factor * lambda * (1 - 2 P)
  * where D[x][y] = ||source_features[x] - target_features[y]||_2 is recomputed
- * on the fly (no n x n distance matrix; features may be NULL for D = 0).
+ * on the fly (no n x n distance matrix; features may be NULL for D = 0);
+ * feature rows hold feature_dim = 4 or 8 floats.
  * regularizer_factor = 1 is sparse_gradient (cugal/pred.py:54-74); 2 is
  * dense_gradient, which adds lambda (1 - 2P) twice (cugal/pred.py:46-50).
  * ld_p: row stride of P in floats (>= n; n for a contiguous P).  The first
@@ -126,8 +141,8 @@ size_t cg_gradient_workspace(int64_t n);
 int64_t cg_gradient_padded_ld(int64_t n);
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
                 int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
-                int lambda_, float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
-                cg_stream_t stream);
+                int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
+                size_t workspace_bytes, cg_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Sinkhorn (log domain, FP32).
@@ -236,7 +251,7 @@ int cg_adjacency_pack_bits(const int32_t* col_indices, const int32_t* row_pointe
                            cg_stream_t stream);
 size_t cg_gradient_mfma_workspace(int64_t n);
 int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, const float* P, int64_t ld_p,
-                     const float* source_features, const float* target_features, int lambda_,
+                     const float* source_features, const float* target_features, int feature_dim, int lambda_,
                      float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
                      cg_stream_t stream);
 

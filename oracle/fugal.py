@@ -68,6 +68,35 @@ def egonet_edges(G: nx.Graph) -> np.ndarray:
     return np.array([nx.ego_graph(G, v).number_of_edges() for v in sorted(G.nodes())], dtype=np.int64)
 
 
+def extract_features_fugal(G: nx.Graph, simple: bool = False) -> np.ndarray:
+    """The node features of the original FUGAL, FUGAL/pred.py:24-110, in
+    sorted node order: the 4 of extract_features_nx and, with simple=False,
+    ego-net edges, edges leaving the ego net (edge tuples of the ego net's
+    nodes not in the ego graph) and distinct nodes at distance 2; n x 7
+    (columns 4-6 zero when simple).
+    Pin: closed forms on K_n, stars, paths and the Petersen graph;
+    nx.edge_boundary of the closed neighbourhood and BFS distances
+    (single_source_shortest_path_length) for the two new counts."""
+    node_list = sorted(G.nodes())
+    deg = dict(G.degree())
+    egonets = {v: nx.ego_graph(G, v) for v in node_list}
+    F = np.zeros((len(node_list), 7))
+    F[:, :4] = extract_features_nx(G)
+    if not simple:
+        F[:, 4] = [egonets[v].number_of_edges() if deg[v] > 0 else 0 for v in node_list]
+        F[:, 5] = [
+            len([edge for edge in set.union(*[set(G.edges(j)) for j in egonets[v].nodes])
+                 if not egonets[v].has_edge(*edge)])
+            for v in node_list
+        ]
+        F[:, 6] = [
+            len(set(p for m in G.neighbors(v) for p in G.neighbors(m)) - set(G.neighbors(v)) - {v})
+            if deg[v] > 0 else 0
+            for v in node_list
+        ]
+    return np.nan_to_num(F)
+
+
 def distance_matrix(source: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
     """torch.cdist of the scaled features, cugal/feature_extraction.py:106-108.
     Pin: direct FP64 Euclidean form (abs tolerance: cdist's mm path
@@ -321,6 +350,7 @@ class OracleConfig:
     sinkhorn_momentum_start: int | None = None
     hungarian_method: str = "DOUBLE_GREEDY"
     quirk_log_marginal: bool = False
+    node_features: str = "cugal"  # or "fugal7": FUGAL/pred.py:24-110, simple=False
 
 
 @dataclass
@@ -343,8 +373,9 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
     Pin: exact recovery of the permutation on noise-free isomorphic pairs
     without automorphisms."""
     pad_graphs(source, target)
-    fs = torch.tensor(extract_features_nx(source), dtype=dtype) * config.mu
-    ft = torch.tensor(extract_features_nx(target), dtype=dtype) * config.mu
+    extract = extract_features_nx if config.node_features == "cugal" else extract_features_fugal
+    fs = torch.tensor(extract(source), dtype=dtype) * config.mu
+    ft = torch.tensor(extract(target), dtype=dtype) * config.mu
     D = distance_matrix(fs, ft)
     A = torch.tensor(nx.to_numpy_array(source, nodelist=sorted(source.nodes())), dtype=dtype)
     B = torch.tensor(nx.to_numpy_array(target, nodelist=sorted(target.nodes())), dtype=dtype)

@@ -70,7 +70,7 @@ def test_config_fields_match_reference():
     for name, value in REFERENCE_FIELDS:
         assert getattr(default, name) == value, name
     # extension fields do not change results by default
-    assert names[len(REFERENCE_FIELDS):] == ["sinkhorn_fused", "gradient_precision"]
+    assert names[len(REFERENCE_FIELDS):] == ["sinkhorn_fused", "gradient_precision", "node_features"]
 
 
 def test_config_enums_match_reference():

@@ -425,3 +425,35 @@ def test_structure_scores_brute_force_and_noise_free_pairs():
     T = nx.to_numpy_array(inst.target, nodelist=range(n))
     assert _scores(S, T, np.arange(n), inst.source_mapping) == (1.0, 1.0, 1.0)
     assert _scores(S, T, inst.source_mapping, inst.source_mapping)[1] < 0.5
+
+
+# ---- FUGAL's 7-feature set (FUGAL/pred.py:24-110) -----------------------------
+
+def test_fugal_features_closed_forms():
+    F = oracle.extract_features_fugal(nx.complete_graph(9))
+    assert np.all(F[:, 4] == 36) and np.all(F[:, 5] == 0) and np.all(F[:, 6] == 0)
+    F = oracle.extract_features_fugal(nx.star_graph(6))  # centre 0, leaves 1..6
+    assert list(F[0, 4:]) == [6, 0, 0]
+    assert np.all(F[1:, 4:] == [1, 5, 5])
+    F = oracle.extract_features_fugal(nx.path_graph(6))
+    assert F[:, 4:].tolist() == [[1, 1, 1], [2, 1, 1], [2, 2, 2], [2, 2, 2], [2, 1, 1], [1, 1, 1]]
+    F = oracle.extract_features_fugal(nx.petersen_graph())  # 3-regular, girth 5
+    assert np.all(F[:, 4:] == [3, 6, 6])
+    g = nx.Graph([(0, 1)])
+    g.add_node(2)
+    assert oracle.extract_features_fugal(g)[2].tolist() == [0.0] * 7
+    simple = oracle.extract_features_fugal(nx.karate_club_graph(), simple=True)
+    np.testing.assert_array_equal(simple[:, :4], oracle.extract_features_nx(nx.karate_club_graph()))
+    assert np.all(simple[:, 4:] == 0)
+
+
+def test_fugal_features_against_networkx_routines():
+    for g in (nx.erdos_renyi_graph(80, 0.08, seed=1), nx.barabasi_albert_graph(90, 3, seed=2),
+              nx.watts_strogatz_graph(70, 6, 0.2, seed=3)):
+        F = oracle.extract_features_fugal(g)
+        for v in sorted(g.nodes()):
+            closed = set(g.neighbors(v)) | {v}
+            assert F[v, 4] == g.subgraph(closed).number_of_edges()
+            assert F[v, 5] == len(list(nx.edge_boundary(g, closed)))
+            dist = nx.single_source_shortest_path_length(g, v, cutoff=2)
+            assert F[v, 6] == sum(1 for d in dist.values() if d == 2)
