@@ -483,3 +483,59 @@ def symmetric_substructure(source_adjacency, target_adjacency, source_mapping, t
         return intersection / (np.sum(source_adjacency == 1) + np.sum(adj2 == 1) - intersection)
     except Exception:
         return 0
+
+
+# ---------------------------------------------------------------------------
+# The original FUGAL (second oracle of the Frank-Wolfe loop)
+# ---------------------------------------------------------------------------
+
+def sinkhorn_knopp_fugal(C: torch.Tensor, reg: float, max_iter: int) -> torch.Tensor:
+    """FUGAL/sinkhorn.py:89-218 as convex_init calls it (marginals of ones,
+    no log, so exactly max_iter iterations from u = v = 1/n): v = 1 / (u^T K),
+    u = 1 / (K v), plan = diag(u) K diag(v).
+    Pin: its plan agrees with solve_standard's after convergence (same fixed
+    point, tests/test_oracle.py)."""
+    n = C.shape[0]
+    K = torch.exp(C / -reg)
+    u = torch.ones(n, dtype=C.dtype) / n
+    v = torch.ones(n, dtype=C.dtype) / n
+    ones = torch.ones(n, dtype=C.dtype)
+    for _ in range(max_iter):
+        v = ones / (u @ K)
+        u = ones / (K @ v)
+    return u.reshape(-1, 1) * K * v.reshape(1, -1)
+
+
+def fugal_original(source: nx.Graph, target: nx.Graph, niter: int, simple: bool, mu: float,
+                   sinkhorn_iterations: int = 500):
+    """Fugal.main + convex_init + convertToPermHungarian (FUGAL/Fugal.py:33-65,
+    FUGAL/pred.py:122-143, :150-164) on graphs labelled 0..n-1: self-loops on
+    isolated rows, features of FUGAL/pred.py:24-110, D = Euclidean distances,
+    for lambda in 0..niter-1 and it in 1..10:
+      G = -A^T P B - A P B^T + mu D + lambda (1 - 2P), q = Sinkhorn(G, reg 1,
+      500 iterations (the reference's maxIter), fresh scalings),
+      P += 2 / (2 + it) (q - P);
+    then scipy's maximising assignment.  Returns (P, list of (row, col))."""
+    n = max(source.number_of_nodes(), target.number_of_nodes())
+    Src = nx.to_numpy_array(source, nodelist=range(n))
+    Tar = nx.to_numpy_array(target, nodelist=range(n))
+    for M in (Src, Tar):
+        for i in range(n):
+            if M[i].sum() == 0:
+                M[i, i] = 1
+    A = torch.tensor(Src, dtype=torch.float64)
+    B = torch.tensor(Tar, dtype=torch.float64)
+    F1 = extract_features_fugal(nx.from_numpy_array(Src), simple)
+    F2 = extract_features_fugal(nx.from_numpy_array(Tar), simple)
+    D = torch.tensor(np.sqrt(np.maximum(((F1[:, None, :] - F2[None, :, :]) ** 2).sum(-1), 0.0)), dtype=torch.float64)
+    P = torch.ones((n, n), dtype=torch.float64) / n
+    ones = torch.ones((n, n), dtype=torch.float64)
+    K = mu * D
+    for i in range(niter):
+        for it in range(1, 11):
+            G = -torch.mm(torch.mm(A.T, P), B) - torch.mm(torch.mm(A, P), B.T) + K + i * (ones - 2 * P)
+            q = sinkhorn_knopp_fugal(G, 1.0, sinkhorn_iterations)
+            alpha = 2.0 / float(2.0 + it)
+            P = P + alpha * (q - P)
+    row, col = scipy.optimize.linear_sum_assignment(P.numpy(), maximize=True)
+    return P, [(int(r), int(c)) for r, c in zip(row, col)]

@@ -457,3 +457,29 @@ def test_fugal_features_against_networkx_routines():
             assert F[v, 5] == len(list(nx.edge_boundary(g, closed)))
             dist = nx.single_source_shortest_path_length(g, v, cutoff=2)
             assert F[v, 6] == sum(1 for d in dist.values() if d == 2)
+
+
+@pytest.mark.parametrize("simple", [True, False])
+def test_cugal_oracle_agrees_with_the_original_fugal(simple):
+    # The original FUGAL's loop (fresh Sinkhorn scalings per step, no
+    # epsilon) and this oracle's cugal loop in the same setting (STANDARD,
+    # warm-started, reg 1, no threshold, lambda term once, SCIPY rounding)
+    # reach the same fixed points once each Sinkhorn solve has converged
+    # (at the reference's 500 iterations the warm and fresh starts still
+    # differ by ~3e-4, at 2000 by ~2e-6): the same assignment and P within
+    # 1e-5.  The 7 features widen the cost range, and Sinkhorn at reg 1
+    # converges far more slowly (2.6e-4 apart at 2000 iterations, 1.5e-4 at
+    # 8000): the same assignment, P within 5e-4.
+    from cugal_amd.synthetic import watts_strogatz
+
+    inst = watts_strogatz(60, 6, 0.2, 0.05, seed=4)
+    src, tgt = inst.source, inst.target
+    assert min(d for _, d in src.degree()) > 0 and min(d for _, d in tgt.degree()) > 0
+    P0, ans0 = oracle.fugal_original(src, tgt, niter=3, simple=simple, mu=0.5, sinkhorn_iterations=2000)
+    run = oracle.align(src.copy(), tgt.copy(), oracle.OracleConfig(
+        sinkhorn_regularization=1.0, sinkhorn_method="STANDARD", sinkhorn_threshold=0.0, sinkhorn_iterations=2000,
+        mu=0.5, iter_count=3,
+        frank_wolfe_iter_count=10, use_sparse_adjacency=True, hungarian_method="SCIPY",
+        node_features="cugal" if simple else "fugal7"))
+    np.testing.assert_allclose(run.P.numpy(), P0.numpy(), rtol=0, atol=1e-5 if simple else 5e-4)
+    assert [c for _, c in run.mapping] == [c for _, c in ans0]
