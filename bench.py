@@ -16,17 +16,23 @@ sums, marginal errors and Frank-Wolfe gap) with the gradient in the form
 the adjacency density calls for (`gradient_precision="auto"`: the CSR
 gathers here, the matrix cores for dense graphs) -- beside the plain FP32
 path (two-sweep Sinkhorn, CSR gradient) on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
-4 n^2 per sweep over the HIP-event time of the sweep launches) and the FP64
+4 n^2 per sweep over the HIP-event time of the sweep launches of every 10th
+iteration, recorded on the launch stream inside the timed run) and the FP64
 CPU oracle timed on a bounded sample of the same Sinkhorn work.  The
 precision form (gradient on the matrix cores: 0/1 BF16 adjacency times BF16
 hi + lo P) runs once on the same input (`precision_form`: time, accuracy,
-matrix-core rate, and the gradient's error against FP64 on sampled rows).
+matrix-core rate, and the gradient's error against FP64 on sampled rows),
+and so does the differentiation form (`differentiation_form`: Erdos-Renyi
+n=4096, 100 unrolled log-Sinkhorn iterations with checkpoints every 10;
+forward and hand-written backward times and the gradients' error against
+torch autograd in FP64).
 
 Other workloads (`--workload`): `ba8192` (Barabasi-Albert n=8192, m=5, 2%
 edge noise; the LAP as greedy and as the exact device Hungarian, checked
 against scipy on the same P) and `sbm65536` (stochastic block model, 64
 blocks of 1024, intra/inter degrees 48/16, 1% noise, 15 lambda stages of one
-Frank-Wolfe step, greedy LAP; P is 17 GB).
+Frank-Wolfe step, greedy LAP; P is 17 GB) and `er4096grad` (the
+differentiation form alone).
 """
 
 from __future__ import annotations

@@ -102,28 +102,32 @@ __device__ __forceinline__ double block_sum_double(double v, double* scratch) {
     return total;
 }
 
-// Feature rows of `dim` floats: 4 (degree, clustering, mean neighbour
-// degree, mean neighbour clustering) or 8 (FUGAL's 7 features and a zero).
+// Feature rows of FD floats: 4 (degree, clustering, mean neighbour degree,
+// mean neighbour clustering) or 8 (FUGAL's 7 features and a zero).  The
+// width is a template parameter so the 4-float epilogues carry no extra
+// address arithmetic or branch per element.
 struct FeatureRow {
     float4 a0, a1;
 };
 
-__device__ __forceinline__ FeatureRow feature_row(const float* __restrict__ f, int dim, int64_t i) {
+template <int FD>
+__device__ __forceinline__ FeatureRow feature_row(const float* __restrict__ f, int64_t i) {
+    static_assert(FD == 4 || FD == 8, "feature rows hold 4 or 8 floats");
     FeatureRow r;
-    r.a0 = *reinterpret_cast<const float4*>(f + dim * i);
-    r.a1 = dim == 8 ? *reinterpret_cast<const float4*>(f + dim * i + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+    r.a0 = *reinterpret_cast<const float4*>(f + FD * i);
+    r.a1 = FD == 8 ? *reinterpret_cast<const float4*>(f + FD * i + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
     return r;
 }
 
 // Euclidean distance between row `a` and row j of ft (the direct form; the
 // first four squares are summed in the same order for either width).
-__device__ __forceinline__ float feature_distance(const FeatureRow& a, const float* __restrict__ ft, int dim,
-                                                 int64_t j) {
-    const float4 b = *reinterpret_cast<const float4*>(ft + dim * j);
+template <int FD>
+__device__ __forceinline__ float feature_distance(const FeatureRow& a, const float* __restrict__ ft, int64_t j) {
+    const float4 b = *reinterpret_cast<const float4*>(ft + FD * j);
     const float d0 = a.a0.x - b.x, d1 = a.a0.y - b.y, d2 = a.a0.z - b.z, d3 = a.a0.w - b.w;
     float s = fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3)));
-    if (dim == 8) {
-        const float4 c = *reinterpret_cast<const float4*>(ft + dim * j + 4);
+    if constexpr (FD == 8) {
+        const float4 c = *reinterpret_cast<const float4*>(ft + FD * j + 4);
         const float e0 = a.a1.x - c.x, e1 = a.a1.y - c.y, e2 = a.a1.z - c.z, e3 = a.a1.w - c.w;
         s += fmaf(e0, e0, fmaf(e1, e1, fmaf(e2, e2, e3 * e3)));
     }

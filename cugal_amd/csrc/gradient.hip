@@ -40,10 +40,11 @@ __device__ __forceinline__ float feat_dist(float4 a, const float* __restrict__ f
     return sqrtf(fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3))));
 }
 
+template <int FD>
 __device__ __forceinline__ float epi_one(float acc, float scale, const Epilogue& ep, int64_t idx, const FeatureRow& fsi,
                                          int j) {
     float g = acc * scale;
-    if (ep.fs) g += feature_distance(fsi, ep.ft, ep.fdim, j);
+    if (ep.fs) g += feature_distance<FD>(fsi, ep.ft, j);
     if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[idx];
     return g;
 }
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
     const int b = arp[i];
     const int e = arp[i + 1];
     FeatureRow fsi{};
-    if (ep.fs) fsi = feature_row(ep.fs, ep.fdim, i);
+    if (ep.fs) fsi = feature_row<4>(ep.fs, i);
     if (VEC) {
         const int j = c0 + threadIdx.x * 4;
         if (j >= n) return;
@@ -81,17 +82,17 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
         }
         const int64_t idx = i * n + j;
         float4 o;
-        o.x = epi_one(acc.x, scale, ep, idx + 0, fsi, j + 0);
-        o.y = epi_one(acc.y, scale, ep, idx + 1, fsi, j + 1);
-        o.z = epi_one(acc.z, scale, ep, idx + 2, fsi, j + 2);
-        o.w = epi_one(acc.w, scale, ep, idx + 3, fsi, j + 3);
+        o.x = epi_one<4>(acc.x, scale, ep, idx + 0, fsi, j + 0);
+        o.y = epi_one<4>(acc.y, scale, ep, idx + 1, fsi, j + 1);
+        o.z = epi_one<4>(acc.z, scale, ep, idx + 2, fsi, j + 2);
+        o.w = epi_one<4>(acc.w, scale, ep, idx + 3, fsi, j + 3);
         *reinterpret_cast<float4*>(out + idx) = o;
     } else {
         for (int j = c0 + threadIdx.x; j < min(n, c0 + chunk); j += kSpThreads) {
             float acc = 0.0f;
             for (int k = b; k < e; ++k) acc += X[static_cast<int64_t>(acol[k]) * n + j];
             const int64_t idx = i * n + j;
-            out[idx] = epi_one(acc, scale, ep, idx, fsi, j);
+            out[idx] = epi_one<4>(acc, scale, ep, idx, fsi, j);
         }
     }
 }
@@ -156,7 +157,7 @@ __device__ __forceinline__ void gather_range(float4& acc, const int* __restrict_
 // Column indices of a tile's rows are staged in LDS in chunks of this many.
 constexpr int kIdxChunk = 2048;
 
-template <bool EPI, bool VEC>
+template <bool EPI, bool VEC, int FD>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
                                                            const float* __restrict__ X, int64_t ldx, float scale,
@@ -238,10 +239,10 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
             FeatureRow fsi{};
-            if (ep.fs) fsi = feature_row(ep.fs, ep.fdim, c);
+            if (ep.fs) fsi = feature_row<FD>(ep.fs, c);
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (rq + e < n) v[e] = epi_one(v[e], scale, ep, pbase + rq + e, fsi, rq + e);
+                if (rq + e < n) v[e] = epi_one<FD>(v[e], scale, ep, pbase + rq + e, fsi, rq + e);
         }
         if (VEC && rq + 3 < n) {
             st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
@@ -292,8 +293,8 @@ __global__ void regularize_kernel(float* __restrict__ g, const float* __restrict
     return cached;
 }
 
-template <bool EPI>
-int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
+template <bool EPI, int FD>
+int launch_gather_tile_fd(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
                        const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
     const bool vec = (n % 4 == 0) && (ldx % 4 == 0) && (ldo % 4 == 0) && aligned16(X) && aligned16(out) &&
                      (!ep.P || aligned16(ep.P)) &&
@@ -303,12 +304,19 @@ int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float
     const int64_t blocks = div_up(tiles, kNumXcd) * kNumXcd;
     if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
     if (vec)
-        hipLaunchKernelGGL((gather_tile_t<EPI, true>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
+        hipLaunchKernelGGL((gather_tile_t<EPI, true, FD>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
                            col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     else
-        hipLaunchKernelGGL((gather_tile_t<EPI, false>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
+        hipLaunchKernelGGL((gather_tile_t<EPI, false, FD>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
                            rp, col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     return check_launch("gather_tile_t");
+}
+
+template <bool EPI>
+int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
+                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
+    if (EPI && ep.fs && ep.fdim == 8) return launch_gather_tile_fd<EPI, 8>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
+    return launch_gather_tile_fd<EPI, 4>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
 }
 
 int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float* X, float scale, const Epilogue& ep,

@@ -151,7 +151,7 @@ __device__ __forceinline__ bool tile_of_block(int64_t bid, int tiles_1d, int& ta
 // left (SPLIT_L: L = Lh + Ll, R = the bit operand) or on the right (L = the
 // bit operand, R = Rh + Rl).  EPI 0: write C as Y_hi / Y_lo; EPI 1: gradient
 // epilogue into G.
-template <bool SPLIT_L, int EPI>
+template <bool SPLIT_L, int EPI, int FD>
 __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__ Xb,  // bit operand, slice-major
                                                    const uint16_t* __restrict__ Sh,  // split operand, hi
                                                    const uint16_t* __restrict__ Sl,  // split operand, lo
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
         for (int r = 0; r < 16; ++r) {
             const int64_t m = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             FeatureRow fsm{};
-            if (EPI == 1 && ep.fs && m < n) fsm = feature_row(ep.fs, ep.fdim, m);
+            if (EPI == 1 && ep.fs && m < n) fsm = feature_row<FD>(ep.fs, m);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int64_t c = c0 + wn * 128 + j * 32 + (lane & 31);
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
                     ep.y_lo[m * np + c] = l;
                 } else if (m < n && c < n) {
                     float g = y;
-                    if (ep.fs) g += feature_distance(fsm, ep.ft, ep.fdim, c);
+                    if (ep.fs) g += feature_distance<FD>(fsm, ep.ft, c);
                     if (ep.P) g += ep.rl - 2.0f * ep.rl * ep.P[m * ep.ldp + c];
                     ep.G[m * n + c] = g;
                 }
@@ -331,19 +331,19 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
     }
 }
 
-template <bool SPLIT_L, int EPI>
+template <bool SPLIT_L, int EPI, int FD>
 int launch_gemm(const uint32_t* X, const uint16_t* Sh, const uint16_t* Sl, int n, int64_t np, float scale,
                 const GradEpi& ep, hipStream_t s, const char* name) {
     static bool attr = false;
     if (!attr) {
-        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_nt<SPLIT_L, EPI>),
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_nt<SPLIT_L, EPI, FD>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, kLdsB));
         attr = true;
     }
     const int tiles_1d = static_cast<int>(np / kTile);
     const int64_t tiles = static_cast<int64_t>(tiles_1d) * tiles_1d;
     const unsigned blocks = static_cast<unsigned>(div_up(tiles, kNumXcd) * kNumXcd);
-    hipLaunchKernelGGL((gemm_nt<SPLIT_L, EPI>), dim3(blocks), dim3(kThreads), kLdsB, s, X, Sh, Sl, n, np, tiles_1d,
+    hipLaunchKernelGGL((gemm_nt<SPLIT_L, EPI, FD>), dim3(blocks), dim3(kThreads), kLdsB, s, X, Sh, Sl, n, np, tiles_1d,
                        scale, ep);
     return check_launch(name);
 }
@@ -414,11 +414,13 @@ int cg_gradient_mfma(const uint32_t* a_bits, const uint32_t* b_bits, int64_t n, 
     hipLaunchKernelGGL(split_transpose, dim3(tb, tb), dim3(256), 0, s, P, ld_p, ni, np, pt_hi, pt_lo);
     if (int rc = check_launch("split_transpose")) return rc;
     GradEpi ep1{nullptr, 0, nullptr, nullptr, 0.0f, nullptr, y_hi, y_lo, 4};
-    if (int rc = launch_gemm<false, 0>(a_bits, pt_hi, pt_lo, ni, np, 1.0f, ep1, s, "gemm_nt<Y = A P>")) return rc;
+    if (int rc = launch_gemm<false, 0, 4>(a_bits, pt_hi, pt_lo, ni, np, 1.0f, ep1, s, "gemm_nt<Y = A P>")) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
     GradEpi ep2{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, gradient, nullptr, nullptr,
                 feature_dim};
-    return launch_gemm<true, 1>(b_bits, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
+    if (source_features && feature_dim == 8)
+        return launch_gemm<true, 1, 8>(b_bits, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
+    return launch_gemm<true, 1, 4>(b_bits, y_hi, y_lo, ni, np, -2.0f, ep2, s, "gemm_nt<G = -2 Y B>");
 }
 
 }  // extern "C"

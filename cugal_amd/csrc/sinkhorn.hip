@@ -1022,6 +1022,15 @@ int timer_mark(hipStream_t s) {
     return 0;
 }
 
+// The timer samples one iteration in kTimerEvery (phase kTimerPhase, never a
+// checking iteration when the check period is kTimerEvery).
+constexpr int kTimerEvery = 10;
+constexpr int kTimerPhase = 5;
+bool timer_sampled(int it) { return it % kTimerEvery == kTimerPhase; }
+int timer_samples_before(int iterations) {
+    return iterations > kTimerPhase ? (iterations - kTimerPhase + kTimerEvery - 1) / kTimerEvery : 0;
+}
+
 // Sum of (end - start) over the first `pairs` recorded pairs (all when
 // negative); call after the stream is synced.  Launches enqueued after the
 // stop exit at once and are left out by the caller's count.
@@ -1227,10 +1236,13 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
             cur = 1;
             for (int it = 0; it < max_iterations && !stopped; ++it) {
                 const bool chk = is_check(it);
-                if (int rc = timer_mark(s)) return rc;
+                const bool timed = timer_sampled(it);
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 if (int rc = launch_fused(C, ni, scale2, w.v2[cur], w.u2, static_cast<float*>(w.part), p, w.ctl, s))
                     return rc;
-                if (int rc = timer_mark(s)) return rc;
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 if (int rc = combine(false, cur, chk ? 1 : 0, it)) return rc;
                 final_default = cur;
                 cur ^= 1;
@@ -1242,15 +1254,20 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
             // column pairs of iteration it+1 (same u, same v).
             for (int it = 0; it < max_iterations && !stopped; ++it) {
                 const bool prev_chk = it > 0 && is_check(it - 1);
-                if (int rc = timer_mark(s)) return rc;
+                const bool timed = timer_sampled(it);
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 if (int rc = launch_col(C, ni, ldc, scale2, w.v2[cur], w.u2, static_cast<float2*>(w.part), p, w.ctl, s))
                     return rc;
-                if (int rc = timer_mark(s)) return rc;
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 if (int rc = combine(true, cur, prev_chk ? 1 : 0, it - 1)) return rc;
                 cur ^= 1;
-                if (int rc = timer_mark(s)) return rc;
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 if (int rc = launch_row(C, ni, ldc, scale2, w.v2[cur], 1.0f, 1.0f, w.u2, 1.0f, w.ctl, s)) return rc;
-                if (int rc = timer_mark(s)) return rc;
+                if (timed)
+                    if (int rc = timer_mark(s)) return rc;
                 final_default = cur;
                 if (prev_chk)
                     if (int rc = post_check()) return rc;
@@ -1258,10 +1275,8 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
             if (!stopped && is_check(max_iterations - 1)) {
                 // The last iteration is a checking one: one more column sweep
                 // to record its error (the returned state is unchanged).
-                if (int rc = timer_mark(s)) return rc;
                 if (int rc = launch_col(C, ni, ldc, scale2, w.v2[cur], w.u2, static_cast<float2*>(w.part), p, w.ctl, s))
                     return rc;
-                if (int rc = timer_mark(s)) return rc;
                 if (int rc = combine(true, cur, 1, max_iterations - 1)) return rc;
             }
         }
@@ -1277,11 +1292,12 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     const int ne = std::min(hc->nerr, errors ? errors_capacity : 0);
     if (errors) std::memcpy(errors, herr, sizeof(double) * static_cast<size_t>(std::max(0, ne)));
     if (error_count) *error_count = hc->nerr;
-    // Timed sweeps that did work: the fused path runs one per iteration up to
-    // the stop; the two-sweep path runs two per iteration plus the column
-    // sweep whose combine raised the stop.
+    // Timed sweeps that did work: those of the sampled iterations before the
+    // stop (one sweep each on the fused path, two on the two-sweep path; the
+    // column sweep whose combine raises the stop belongs to an iteration that
+    // is a multiple of the check period, never sampled).
     int64_t worked = -1;
-    if (hc->done) worked = use_fused ? hc->iterations : 2 * static_cast<int64_t>(hc->iterations) + 1;
+    if (hc->done) worked = (use_fused ? 1 : 2) * static_cast<int64_t>(timer_samples_before(hc->iterations));
     if (int rc = timer_total(sweep_ms, sweep_launches, worked)) return rc;
     return CG_OK;
 }

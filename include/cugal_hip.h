@@ -176,8 +176,9 @@ int cg_sinkhorn_log_marginal(const float* K, int64_t n, const float* u, const fl
  * Synchronises the stream once at the end.  Host outputs: errors[0 ..
  * min(count, errors_capacity)), *error_count, *iterations (the reference's
  * SinkhornProfile.iteration_count), and when non-NULL the summed duration and
- * number of the C-reading sweep kernels (for roofline reporting; adds event
- * records between launches). */
+ * number of the C-reading sweep kernels of every 10th iteration (iterations
+ * 5, 15, 25, ...; for roofline reporting: event records around those
+ * launches only, so the timer costs ~1/10 of timing every sweep). */
 size_t cg_sinkhorn_workspace(int64_t n, int fused, int max_iterations, int eval_freq);
 int cg_sinkhorn_fused_supported(int64_t n);
 int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float* u, float* v, int max_iterations,

@@ -193,3 +193,28 @@ def test_reference_step_and_marginal(device):
     got = marginal_error_log(Kd.clone(), u.float().to(device), v.float().to(device))
     want = float((torch.exp(K.float().double() + v.float().double()[None, :] + u.float().double()[:, None]).sum(1) - 1).abs().sum())
     assert abs(got - want) <= 1e-4 * want
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_sweep_timer_samples_and_changes_nothing(device, fused):
+    # The sweep timer records events around the sweeps of iterations 5, 15,
+    # 25, ... only; it must not change the iterates, and it counts only the
+    # sampled sweeps that ran before the stop.
+    from cugal_amd.sinkhorn import SweepTiming, solve_log_native
+
+    n = 512
+    C = (torch.rand((n, n), generator=torch.Generator().manual_seed(9)) * 4).to(device)
+    runs = []
+    for timing in (False, True):
+        u = torch.zeros(n, device=device)
+        v = torch.zeros(n, device=device)
+        SweepTiming.reset(timing)
+        errors, iters, _ = solve_log_native(C, 0.3, u, v, 500, 1e-3, 10, None, fused=fused)
+        runs.append((u.cpu(), v.cpu(), iters, errors, SweepTiming.total_ms, SweepTiming.launches))
+        SweepTiming.reset(False)
+    (u0, v0, it0, e0, _, _), (u1, v1, it1, e1, ms, launches) = runs
+    assert torch.equal(u0, u1) and torch.equal(v0, v1) and it0 == it1 and e0 == e1
+    assert 10 < it0 < 500, it0
+    sampled = (it0 - 5 + 9) // 10
+    assert launches == sampled * (1 if fused else 2), (launches, it0)
+    assert ms > 0.0
