@@ -94,3 +94,45 @@ def test_safe_mode_and_config_errors(device):
     with pytest.raises(NotImplementedError):
         cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target),
               Config(**{**gpu.__dict__, "sinkhorn_method": SinkhornMethod.MIX}), Profile())
+
+
+@pytest.mark.parametrize("form", ["log_fused_csr", "log_plain_csr", "standard_fused", "log_bf16x2_dense_lap"])
+def test_alignment_is_deterministic(device, form):
+    # Every reduction runs in a fixed order (no floating-point atomics), so
+    # two runs on the same inputs give bitwise-equal P and the same mapping.
+    from cugal_amd.pred import align_device
+    from cugal_amd.pred import _adjacency_of
+
+    inst = erdos_renyi(500, 0.02, 0.05, seed=8)
+    gpu, _ = _configs(True, 0.3, 2.0, 3, 4, fused=form != "log_plain_csr")
+    if form == "standard_fused":
+        gpu = Config(**{**gpu.__dict__, "sinkhorn_method": SinkhornMethod.STANDARD, "sinkhorn_regularization": 1.0})
+    if form == "log_bf16x2_dense_lap":
+        gpu = Config(**{**gpu.__dict__, "gradient_precision": "bf16x2", "hungarian_method": HungarianMethod.DENSE})
+    src, tgt = copy.deepcopy(inst.source), copy.deepcopy(inst.target)
+    node_count = max(max(src.nodes()), max(tgt.nodes()))
+    src.add_nodes_from(range(node_count))
+    tgt.add_nodes_from(range(node_count))
+    A, B = _adjacency_of(src, gpu), _adjacency_of(tgt, gpu)
+    P1, c1 = align_device(A, B, gpu)
+    P1 = P1.clone()
+    P2, c2 = align_device(A, B, gpu)
+    assert torch.equal(P1, P2)
+    assert torch.equal(torch.as_tensor(c1).cpu(), torch.as_tensor(c2).cpu())
+
+
+def test_safe_mode_reports_a_non_finite_plan(device):
+    # Forced from the host: complete graphs give a cost near -2n, so at
+    # reg 0.01 the STANDARD kernel exp(-C/reg) overflows FP32 to inf and the
+    # plan becomes NaN (0 * inf), as in the reference; safe_mode stops with
+    # the reference's message before the LAP.
+    import networkx as nx
+
+    g = nx.complete_graph(30)
+    perm = np.random.default_rng(0).permutation(30)
+    h = nx.relabel_nodes(g, {i: int(perm[i]) for i in range(30)})
+    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.STANDARD,
+                 sinkhorn_regularization=0.01, sinkhorn_iterations=20, iter_count=1, frank_wolfe_iter_count=1,
+                 use_sparse_adjacency=False, safe_mode=True)
+    with pytest.raises(AssertionError, match="quasi_permutation tensor has NaN values"):
+        cugal(g, h, cfg, Profile())
