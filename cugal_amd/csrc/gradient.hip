@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
 }
 
 // ---- tiled gather with a transposed store ---------------------------------
-// For the 64 x 32 tile (rows r0.., columns c0..) of Z = M X (M given by CSR):
+// For the 64 x TC tile (rows r0.., columns c0..) of Z = M X (M given by CSR):
 //   Z[r][c] = sum_{k in N(r)} X[k][c],
 // write out[c][r] = Z[r][c] (EPI == false), or, with EPI, treat (c, r) as
 // (i, j) of the gradient: out[c][r] = scale * Z[r][c] + D(c, r) + rl (1 - 2 P[c][r]).
@@ -108,7 +108,6 @@ __global__ __launch_bounds__(kSpThreads) void spmm_rowgather(const int32_t* __re
 // every row that lists it; outputs are stored non-temporally so they do not
 // evict the strip.
 constexpr int kTileR = 64;
-constexpr int kTileC = 32;
 
 __device__ __forceinline__ void st_stream4(float* p, float4 v) {
     f32x4 w = {v.x, v.y, v.z, v.w};
@@ -157,59 +156,61 @@ __device__ __forceinline__ void gather_range(float4& acc, const int* __restrict_
 // Column indices of a tile's rows are staged in LDS in chunks of this many.
 constexpr int kIdxChunk = 2048;
 
-template <bool EPI, bool VEC, int FD>
+template <bool EPI, bool VEC, int FD, int TC>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
                                                            const float* __restrict__ X, int64_t ldx, float scale,
                                                            Epilogue ep, float* __restrict__ out, int64_t ldo,
                                                            int tiles_r, int64_t tiles) {
-    __shared__ float tile[kTileR][kTileC + 1];
+    constexpr int LPT = TC / 4;              // threads per TC-float segment
+    constexpr int RG = kSpThreads / LPT;     // row groups
+    constexpr int RPT = kTileR / RG;         // rows per thread
+    __shared__ float tile[kTileR][TC + 1];
     __shared__ int sidx[VEC ? kIdxChunk : 1];
     __shared__ int srp[kTileR + 1];
     const int64_t t = xcd_tile(blockIdx.x, tiles);
     if (t >= tiles) return;
     const int r0 = static_cast<int>(t % tiles_r) * kTileR;
-    const int c0 = static_cast<int>(t / tiles_r) * kTileC;
+    const int c0 = static_cast<int>(t / tiles_r) * TC;
     if (VEC) {
         // The tile's index lists, staged through LDS with coalesced loads, so
         // each gather step issues its row loads without a dependent index
         // load first (measured 11 % faster per gradient at n = 32768).
-        const int g = threadIdx.x >> 3;  // 32 row groups of 8 threads
-        const int q = threadIdx.x & 7;   // float4 slot within a 32-float line
+        const int g = threadIdx.x / LPT;  // row group
+        const int q = threadIdx.x % LPT;  // float4 slot within the segment
         const int cq = c0 + 4 * q;
         if (threadIdx.x <= kTileR) srp[threadIdx.x] = rp[min(r0 + static_cast<int>(threadIdx.x), n)];
         __syncthreads();
         const int e_lo = srp[0];
         const int e_hi = srp[kTileR];
-        float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int cs = e_lo; cs < e_hi; cs += kIdxChunk) {
             const int ce = min(e_hi, cs + kIdxChunk);
             for (int k = cs + static_cast<int>(threadIdx.x); k < ce; k += kSpThreads) sidx[k - cs] = col[k];
             __syncthreads();
             if (cq < n) {
-                const int b0 = max(srp[g], cs), e0 = min(srp[g + 1], ce);
-                const int b1 = max(srp[g + 32], cs), e1 = min(srp[g + 33], ce);
-                // One row after the other: gathering both at once measured
-                // slower (more registers, fewer waves in flight).
-                gather_range(acc0, sidx, cs, b0, e0, X, ldx, cq);
-                gather_range(acc1, sidx, cs, b1, e1, X, ldx, cq);
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    const int rr = g + r * RG;
+                    gather_range(acc[r], sidx, cs, max(srp[rr], cs), min(srp[rr + 1], ce), X, ldx, cq);
+                }
             }
             __syncthreads();
         }
-        tile[g][4 * q + 0] = acc0.x;
-        tile[g][4 * q + 1] = acc0.y;
-        tile[g][4 * q + 2] = acc0.z;
-        tile[g][4 * q + 3] = acc0.w;
-        tile[g + 32][4 * q + 0] = acc1.x;
-        tile[g + 32][4 * q + 1] = acc1.y;
-        tile[g + 32][4 * q + 2] = acc1.z;
-        tile[g + 32][4 * q + 3] = acc1.w;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            tile[g + r * RG][4 * q + 0] = acc[r].x;
+            tile[g + r * RG][4 * q + 1] = acc[r].y;
+            tile[g + r * RG][4 * q + 2] = acc[r].z;
+            tile[g + r * RG][4 * q + 3] = acc[r].w;
+        }
     } else {
-        const int g = threadIdx.x >> 3;
-        const int q = threadIdx.x & 7;
+        const int g = threadIdx.x / LPT;
+        const int q = threadIdx.x % LPT;
         const int cq = c0 + 4 * q;
-        for (int rr = g; rr < kTileR; rr += 32) {
+        for (int rr = g; rr < kTileR; rr += RG) {
             const int r = r0 + rr;
             float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
             if (r < n) {
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
     const int g = threadIdx.x >> 4;  // 16 output rows per pass
     const int q = threadIdx.x & 15;  // four consecutive output columns
     const int rq = r0 + 4 * q;
-    for (int cc = g; cc < kTileC; cc += 16) {
+    for (int cc = g; cc < TC; cc += 16) {
         const int c = c0 + cc;
         if (c >= n) break;
         float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
@@ -293,21 +294,21 @@ __global__ void regularize_kernel(float* __restrict__ g, const float* __restrict
     return cached;
 }
 
-template <bool EPI, int FD>
+template <bool EPI, int FD, int TC>
 int launch_gather_tile_fd(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
                        const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
     const bool vec = (n % 4 == 0) && (ldx % 4 == 0) && (ldo % 4 == 0) && aligned16(X) && aligned16(out) &&
                      (!ep.P || aligned16(ep.P)) &&
                      (!ep.fs || (aligned16(ep.fs) && aligned16(ep.ft)));
     const int tiles_r = static_cast<int>(div_up(n, kTileR));
-    const int64_t tiles = static_cast<int64_t>(tiles_r) * div_up(n, kTileC);
+    const int64_t tiles = static_cast<int64_t>(tiles_r) * div_up(n, TC);
     const int64_t blocks = div_up(tiles, kNumXcd) * kNumXcd;
     if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
     if (vec)
-        hipLaunchKernelGGL((gather_tile_t<EPI, true, FD>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
+        hipLaunchKernelGGL((gather_tile_t<EPI, true, FD, TC>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
                            col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     else
-        hipLaunchKernelGGL((gather_tile_t<EPI, false, FD>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
+        hipLaunchKernelGGL((gather_tile_t<EPI, false, FD, TC>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
                            rp, col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
     return check_launch("gather_tile_t");
 }
@@ -315,8 +316,13 @@ int launch_gather_tile_fd(const int32_t* rp, const int32_t* col, int n, const fl
 template <bool EPI>
 int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
                        const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
-    if (EPI && ep.fs && ep.fdim == 8) return launch_gather_tile_fd<EPI, 8>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
-    return launch_gather_tile_fd<EPI, 4>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
+    // The first product (no epilogue) gathers 64-column strips: 256-byte
+    // segments, 4 % faster there; the second gathers 32-column strips, whose
+    // 4 MiB footprint at n = 32768 already fills an XCD's L2 (64 columns
+    // measured 4 % slower).  Same sums in the same order either way.
+    if (!EPI) return launch_gather_tile_fd<EPI, 4, 64>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
+    if (ep.fs && ep.fdim == 8) return launch_gather_tile_fd<EPI, 8, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
+    return launch_gather_tile_fd<EPI, 4, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
 }
 
 int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float* X, float scale, const Epilogue& ep,
