@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kSapThreads) void sap_solve(const float* __restrict
     const int nfree = s_nfree;
 
     uint32_t freebits = 0;
-#pragma unroll(kUnroll)
+#pragma unroll kUnroll
     for (int q = 0; q < Q; ++q) {
         const int j = tid + q * kSapThreads;
         if (j < n && row4col[j] < 0) freebits |= 1u << q;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kSapThreads) void sap_solve(const float* __restrict
     bool failed = false;
     for (int f = 0; f < nfree && !failed; ++f) {
         const int i0 = free_rows[f];
-#pragma unroll(kUnroll)
+#pragma unroll kUnroll
         for (int q = 0; q < Q; ++q) {
             const int j = tid + q * kSapThreads;
             if (R || j < n) d.set(q, j, INFINITY);
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kSapThreads) void sap_solve(const float* __restrict
             const float* row = C + static_cast<int64_t>(i) * n;
             double bd = INFINITY;
             int bk = kNoRow;
-#pragma unroll(kUnroll)
+#pragma unroll kUnroll
             for (int q = 0; q < Q; ++q) {
                 const int j = tid + q * kSapThreads;
                 if (j < n && !((scanned >> q) & 1u)) {
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kSapThreads) void sap_solve(const float* __restrict
                     }
                 }
             }
-#pragma unroll(kUnroll)
+#pragma unroll kUnroll
             for (int off = 32; off > 0; off >>= 1) {
                 const double od = __shfl_xor(bd, off, kWave);
                 const int ok = __shfl_xor(bk, off, kWave);
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kSapThreads) void sap_solve(const float* __restrict
         }
         // Dual update over the scanned columns and the rows reached through
         // them, then flip the path.
-#pragma unroll(kUnroll)
+#pragma unroll kUnroll
         for (int q = 0; q < Q; ++q) {
             if ((scanned >> q) & 1u) {
                 const int j = tid + q * kSapThreads;
