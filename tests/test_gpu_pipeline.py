@@ -82,7 +82,10 @@ def test_fixed_iteration_quasi_permutation_matches_oracle(device):
     ref.sinkhorn_iterations = 30
     P, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, Profile())
     run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)
-    np.testing.assert_allclose(P, run.P.numpy(), rtol=1e-3, atol=1e-6)
+    # Q within 1e-5 relative of the FP64 oracle (measured 1.4e-6), entries
+    # below 1e-6 of the largest to 1e-5 of that floor.
+    R = run.P.numpy()
+    np.testing.assert_allclose(P, R, rtol=1e-5, atol=1e-11 * float(R.max()))
     assert [m for _, m in mapping] == [m for _, m in run.mapping]
 
 

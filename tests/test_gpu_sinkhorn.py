@@ -38,9 +38,9 @@ def _run_oracle(C64, reg, iters, thr, freq, u0=None, v0=None, momentum_start=Non
 
 # FP32 log-domain iterates against FP64: |C/reg| <= 33 here, so potentials
 # carry about 33 * 2^-24 ~ 2e-6 absolute rounding per step; 5e-5 absolute on
-# u, v and 1e-4 relative on plan entries bound the accumulated drift.
+# u, v bounds the accumulated drift (measured 6e-7, or 7e-6 on the two-sweep
+# path where it is a gauge shift that cancels in the plan).
 U_ATOL = 5e-5
-PLAN_RTOL = 1e-4
 
 
 def error_floor(n, C, reg):
@@ -71,7 +71,12 @@ def test_fixed_iterations_match_oracle(device, n, fused):
     np.testing.assert_allclose(v.numpy(), rv.numpy(), atol=U_ATOL * max(1.0, 1.0 / reg))
     plan = oracle.plan_log(K, u, v)
     ref_plan = oracle.plan_log(K, ru, rv)
-    np.testing.assert_allclose(plan.numpy(), ref_plan.numpy(), rtol=PLAN_RTOL * 10, atol=1e-9)
+    # The plan within 1e-5 relative of FP64 (measured 5e-7 to 8e-7; the
+    # two-sweep path's potentials carry a 7e-6 gauge shift u + c, v - c that
+    # cancels in the plan).  Entries below 1e-6 of the largest are held to
+    # 1e-5 of that floor in absolute terms.
+    floor = 1e-6 * float(ref_plan.max())
+    np.testing.assert_allclose(plan.numpy(), ref_plan.numpy(), rtol=1e-5, atol=1e-5 * floor)
     # errors recorded at iterations 10, 20, 30 (not 0)
     assert len(errors) == len(res.errors) == 3
     assert_errors_close(errors, res.errors, n, C, reg)
