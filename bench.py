@@ -106,11 +106,13 @@ def api_extra_seconds(inst, P, device) -> float:
     and the host copy of P."""
     import torch
 
+    from cugal_amd.pred import to_host
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     device_inputs(inst, device)
     torch.cuda.synchronize()
-    P.cpu().numpy()
+    to_host(P)
     return time.perf_counter() - t0
 
 

@@ -139,3 +139,27 @@ def test_safe_mode_reports_a_non_finite_plan(device):
                  use_sparse_adjacency=False, safe_mode=True)
     with pytest.raises(AssertionError, match="quasi_permutation tensor has NaN values"):
         cugal(g, h, cfg, Profile())
+
+
+@pytest.mark.parametrize("method", ["LOG", "STANDARD"])
+def test_pipeline_at_an_odd_size(device, method):
+    # n = 333: the fused Sinkhorn sweep, the float4 gathers and the float4
+    # update all fall back to their scalar forms; same accuracy as the oracle.
+    import networkx as nx
+
+    from cugal_amd.synthetic import NoiseLevel, make_instance
+
+    inst = make_instance(nx.erdos_renyi_graph(333, 0.03, seed=6), NoiseLevel(0.0, 0.05, True), seed=6)
+    src, tgt = copy.deepcopy(inst.source), copy.deepcopy(inst.target)
+    reg = 0.3 if method == "LOG" else 1.0
+    gpu = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod[method],
+                 sinkhorn_regularization=reg, mu=2.0, iter_count=4, frank_wolfe_iter_count=4,
+                 use_sparse_adjacency=True)
+    ref = oracle.OracleConfig(sinkhorn_regularization=reg, sinkhorn_method=method, mu=2.0, iter_count=4,
+                              frank_wolfe_iter_count=4, use_sparse_adjacency=True)
+    P, mapping = cugal(src, tgt, gpu, Profile())
+    assert P.shape[0] % 4 != 0, P.shape
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)
+    acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
+    ref_acc = alignment_accuracy(inst.source_mapping, answer_columns(run.mapping))
+    assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
