@@ -969,13 +969,16 @@ int launch_std_row(const float* C, int n, float scale2, const float* v, float* u
     return check_launch("std_row_sweep");
 }
 
-// Pinned host mirror of the control block, grown on demand.
+// Pinned host mirror of the control block, grown on demand, and the pinned
+// slots the checking combines write their stop decisions into.  Both are
+// per host thread, so solves issued from different threads (on different
+// streams) do not share them; they are kept until the process ends.
 struct HostMirror {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
-HostMirror g_mirror;
-int* g_done_slots = nullptr;  // pinned poll slots
+thread_local HostMirror g_mirror;
+thread_local int* g_done_slots = nullptr;  // pinned poll slots
 
 int ensure_pinned(size_t bytes) {
     if (!g_done_slots) CG_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_done_slots), 64 * sizeof(int), hipHostMallocDefault));
@@ -995,7 +998,7 @@ struct SweepTimer {
     size_t used = 0;
     int device = -1;
 };
-SweepTimer g_timer;
+thread_local SweepTimer g_timer;  // per host thread, like the poll state
 
 int timer_begin(bool on) {
     g_timer.on = on;

@@ -17,7 +17,10 @@
  *     matching *_workspace() function returns; the library never allocates
  *     device memory;
  *   - the return value is 0 on success or a negative code, with a message
- *     available from cg_last_error() (thread-local).
+ *     available from cg_last_error() (thread-local);
+ *   - calls from several host threads may run at once when each uses its
+ *     own workspace (and stream): host-side state such as the Sinkhorn
+ *     solves' pinned poll slots is per thread.
  *
  * Each declaration names the reference interface it replaces
  * (SebastianRueClausen/cuGAL, path:line).

@@ -223,3 +223,49 @@ def test_sweep_timer_samples_and_changes_nothing(device, fused):
     sampled = (it0 - 5 + 9) // 10
     assert launches == sampled * (1 if fused else 2), (launches, it0)
     assert ms > 0.0
+
+
+def test_concurrent_solves_from_two_threads(device):
+    # Two host threads, each with its own stream and workspace, run
+    # threshold-stopped solves through the C ABI at the same time; the stop
+    # flags and control mirrors are per thread, so each gets exactly what it
+    # gets alone (iterates bitwise, same iteration count).
+    import threading
+
+    import numpy as np
+
+    def solve(C, reg, stream):
+        n = C.shape[0]
+        with torch.cuda.stream(stream):
+            u = torch.zeros(n, device=device)
+            v = torch.zeros(n, device=device)
+            ws = torch.empty(int(nat.query("cg_sinkhorn_workspace", n, 1, 500, 10)), dtype=torch.uint8,
+                             device=device)
+            cap = 500 // 10 + 4
+            errors = np.zeros(cap)
+            count = np.zeros(1, dtype=np.int32)
+            iters = np.zeros(1, dtype=np.int32)
+            for _ in range(3):  # several solves, warm-started, to overlap many checks
+                nat.call("cg_sinkhorn_log_solve", nat.ptr(C), n, float(reg), nat.ptr(u), nat.ptr(v), 500, 1e-3, 10,
+                         -1, 1, errors.ctypes.data, cap, count.ctypes.data, iters.ctypes.data, None, None,
+                         nat.ptr(ws), ws.numel(), nat.stream_of(C.device))
+            stream.synchronize()
+            return u.cpu(), v.cpu(), int(iters[0])
+
+    costs = [(_cost(2048, seed=31, scale=2.0).float().to(device), 0.5),
+             (_cost(2048, seed=32, scale=6.0).float().to(device), 0.3)]
+    alone = [solve(C, reg, torch.cuda.Stream(device)) for C, reg in costs]
+    results = [None, None]
+
+    def worker(k):
+        results[k] = solve(costs[k][0], costs[k][1], torch.cuda.Stream(device))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for k in range(2):
+        assert results[k] is not None
+        assert torch.equal(results[k][0], alone[k][0]) and torch.equal(results[k][1], alone[k][1])
+        assert results[k][2] == alone[k][2] and 10 <= alone[k][2] < 500
