@@ -3,11 +3,12 @@
 ``cugal(source, target, config, profile)`` pads the two graphs to one node
 range, builds their CSR adjacency and structural features on the device,
 runs ``iter_count`` lambda stages of ``frank_wolfe_iter_count`` Frank-Wolfe
-steps (gradient -> log-Sinkhorn -> quasi-permutation update) and rounds the
+steps (gradient -> Sinkhorn -> quasi-permutation update) and rounds the
 result with the configured LAP.  Every n x n pass runs in the HIP kernels of
-libcugal_hip.so; the host only sequences them.  Supported: config.device a
-ROCm device ("cuda"), dtype torch.float32, SinkhornMethod.LOG, undirected
-graphs.  Anything else raises rather than falling back.
+libcugal_hip.so; the host only sequences them (the SCIPY and SPARSE LAPs run
+scipy on the host, as the reference does).  Supported: config.device a ROCm
+device ("cuda"), dtype torch.float32, SinkhornMethod.LOG or STANDARD,
+undirected graphs.  Anything else raises rather than falling back.
 """
 
 from __future__ import annotations
