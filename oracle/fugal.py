@@ -416,7 +416,7 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
 
 
 # ---------------------------------------------------------------------------
-# Differentiable references (BASELINE differentiation form)
+# Differentiable references: unrolled log-domain Sinkhorn and its loss
 # ---------------------------------------------------------------------------
 
 def unrolled_plan(C: torch.Tensor, reg: float, iterations: int, u0=None, v0=None) -> torch.Tensor:

@@ -3,6 +3,7 @@
 
 import copy
 
+import networkx as nx
 import numpy as np
 import pytest
 import torch
@@ -163,3 +164,53 @@ def test_pipeline_at_an_odd_size(device, method):
     acc = alignment_accuracy(inst.source_mapping, answer_columns(mapping))
     ref_acc = alignment_accuracy(inst.source_mapping, answer_columns(run.mapping))
     assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
+
+
+
+_TINY = {
+    # one edge: the two nodes are interchangeable, so P stays uniform and the
+    # LAP's lowest-index rule decides
+    "edge": (lambda: nx.Graph([(0, 1)]), [1, 0]),
+    # a tree with branches of lengths 1, 2 and 3 at node 2: no automorphism
+    "asymmetric_tree": (lambda: nx.Graph([(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6)]), [4, 0, 6, 2, 5, 3, 1]),
+    # the same tree on labels 0-2, 4-7: cugal() pads the source with node 3
+    # and the target with node 2 (n = 8, the fused Sinkhorn width)
+    "padded_tree": (lambda: nx.Graph([(0, 1), (1, 2), (2, 4), (4, 5), (5, 6), (2, 7)]), [5, 7, 0, 2, 6, 1, 4, 3]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(_TINY))
+@pytest.mark.parametrize("method,lap", [("LOG", "DOUBLE_GREEDY"), ("STANDARD", "GREEDY"), ("LOG", "SCIPY"),
+                                        ("STANDARD", "DENSE")])
+def test_tiny_graphs_match_oracle(device, name, method, lap):
+    # Graphs of 2 to 7 nodes: every kernel runs far below its tile size
+    # (scalar fallbacks, partial wavefronts).  With a zero threshold each
+    # solve runs its full count, so Q follows the FP64 oracle entry by entry
+    # and the rounded mapping is the oracle's; DENSE (the exact device LAP)
+    # is held to scipy's optimum, since a tie may have several optimal
+    # assignments.
+    make, perm = _TINY[name]
+    source = make()
+    source.remove_edges_from(list(nx.selfloop_edges(source)))
+    target = nx.relabel_nodes(source, {i: perm[i] for i in source.nodes()})
+    reg = 0.3 if method == "LOG" else 1.0
+    gpu = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod[method],
+                 sinkhorn_regularization=reg, sinkhorn_threshold=0.0, sinkhorn_iterations=40, mu=1.0, iter_count=3,
+                 frank_wolfe_iter_count=3, use_sparse_adjacency=True, hungarian_method=HungarianMethod[lap])
+    ref = oracle.OracleConfig(sinkhorn_regularization=reg, sinkhorn_method=method, sinkhorn_threshold=0.0, mu=1.0,
+                              iter_count=3, frank_wolfe_iter_count=3, use_sparse_adjacency=True,
+                              hungarian_method="SCIPY" if lap == "DENSE" else lap)
+    ref.sinkhorn_iterations = 40
+    P, mapping = cugal(copy.deepcopy(source), copy.deepcopy(target), gpu, Profile())
+    run = oracle.align(copy.deepcopy(source), copy.deepcopy(target), ref)
+    R = run.P.numpy()
+    assert P.shape == R.shape
+    np.testing.assert_allclose(P, R, rtol=1e-5, atol=1e-6 * float(R.max()))
+    cols = [m for _, m in mapping]
+    ref_cols = [m for _, m in run.mapping]
+    if lap == "DENSE":
+        assert sorted(cols) == list(range(len(cols)))
+        rows = np.arange(len(cols))
+        assert abs(R[rows, cols].sum() - R[rows, ref_cols].sum()) <= 1e-6 * len(cols)
+    else:
+        assert cols == ref_cols
