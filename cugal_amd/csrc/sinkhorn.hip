@@ -96,14 +96,27 @@ __global__ void sk_finish(const float* __restrict__ u2, const float* __restrict_
 // is reduced (two register buffers).
 // ---------------------------------------------------------------------------
 // FULL: n == 4 G T, every column group exists (no bounds tests).
-template <int T, int G, bool FULL = false>
+// KEEP: caching loads instead of non-temporal ones (see kCacheableMaxN).
+template <int T, int G, bool FULL = false, bool KEEP = false>
 __device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __restrict__ row, int n) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + threadIdx.x) * 4;
-        r[g] = (FULL || col < n) ? ld_stream4(row + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+        r[g] = (FULL || col < n) ? (KEEP ? *reinterpret_cast<const float4*>(row + col) : ld_stream4(row + col))
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
+
+// Up to this width (a 576 MiB matrix) the fused sweeps read C with caching
+// loads, and the LOG sweep walks its rows backwards on every other
+// iteration, so a sweep starts on the rows the previous one read last,
+// which the L2s and the 256 MB memory-side cache still hold.  Measured per
+// LOG iteration: 11 % less time at n = 4096, 8 % at 8192, 2-3 % at 12288;
+// even at 16384 and 9 % more at 32768, where non-temporal loads stream
+// faster.  The STANDARD sweep keeps its forward order; caching loads alone
+// take 10 % off at 4096 and 8 % at 8192, nothing at 12288.
+constexpr int kCacheableMaxN = 12288;
+constexpr int kStdCacheableMaxN = 8192;
 
 template <int T, int G, bool FULL = false>
 __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], int i, int n, float scale2,
@@ -217,10 +230,10 @@ __device__ __forceinline__ bool fused_row_shift(float4 (&r)[G], float (&cs)[4 * 
 constexpr int kFusedMaxRowsPerBlock = 128;  // n <= 32768 over 256 blocks
 
 
-template <int T, int G, bool FULL>
+template <int T, int G, bool FULL, bool KEEP>
 __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C, int n, float scale2,
                                                    const float* __restrict__ v2, float* __restrict__ u2,
-                                                   float* __restrict__ part, int rows_per_blk,
+                                                   float* __restrict__ part, int rows_per_blk, int reverse,
                                                    const SinkCtl* __restrict__ ctl) {
     extern __shared__ float smem[];
     __shared__ int queued[kFusedMaxRowsPerBlock];
@@ -248,12 +261,15 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
 
     const int64_t ld = n;
     // One row buffer: a second one (loading row k+1 during the reduction of
-    // row k) measured slower; the 16 waves' loads keep HBM busy.
-    for (int k = 0; k < rows; ++k) {
+    // row k) measured slower; the 16 waves' loads keep HBM busy.  `reverse`
+    // walks the block's rows last to first (the column sums then add in
+    // that order).
+    for (int q = 0; q < rows; ++q) {
+        const int k = reverse ? rows - 1 - q : q;
         const int i = static_cast<int>(blockIdx.x) + k * nblk;
         float4 a[G];
-        fused_load_row<T, G, FULL>(a, C + i * ld, n);
-        if (!fused_row_shift<T, G, FULL>(a, cs, i, k & 1, n, scale2, sv, red, u2, su[k], mom) && tid == 0)
+        fused_load_row<T, G, FULL, KEEP>(a, C + i * ld, n);
+        if (!fused_row_shift<T, G, FULL>(a, cs, i, q & 1, n, scale2, sv, red, u2, su[k], mom) && tid == 0)
             queued[nqueued++] = i;
     }
 
@@ -573,7 +589,7 @@ __device__ __forceinline__ void std_fused_row(float4 (&r)[G], float (&cs)[4 * G]
     }
 }
 
-template <int T, int G>
+template <int T, int G, bool KEEP>
 __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C, int n, float scale2,
                                                     const float* __restrict__ v, float* __restrict__ u,
                                                     float* __restrict__ part, int rows_per_blk,
@@ -595,18 +611,18 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
     const int64_t ld = n;
     if constexpr (4 * G * T <= 24 * 1024) {
         float4 a[G], b[G];
-        if (r0 < r1) fused_load_row<T, G>(a, C + r0 * ld, n);
+        if (r0 < r1) fused_load_row<T, G, false, KEEP>(a, C + r0 * ld, n);
         for (int i = r0; i < r1; i += 2) {
-            if (i + 1 < r1) fused_load_row<T, G>(b, C + (i + 1) * ld, n);
+            if (i + 1 < r1) fused_load_row<T, G, false, KEEP>(b, C + (i + 1) * ld, n);
             std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
             if (i + 1 >= r1) break;
-            if (i + 2 < r1) fused_load_row<T, G>(a, C + (i + 2) * ld, n);
+            if (i + 2 < r1) fused_load_row<T, G, false, KEEP>(a, C + (i + 2) * ld, n);
             std_fused_row<T, G>(b, cs, i + 1, n, scale2, sv, red, u, mom);
         }
     } else {
         float4 a[G];
         for (int i = r0; i < r1; ++i) {
-            fused_load_row<T, G>(a, C + i * ld, n);
+            fused_load_row<T, G, false, KEEP>(a, C + i * ld, n);
             std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
         }
     }
@@ -845,9 +861,17 @@ int max_lds_bytes() {
     return cached;
 }
 
+template <int T, int G, bool FULL, bool KEEP>
+hipError_t set_fused_lds(size_t lds) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G, FULL, KEEP>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
+// `reverse`: the iteration's parity; the row order alternates only where
+// the matrix is read with caching loads.
 template <int T, int G>
 int launch_fused_tg(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
-                    const SinkCtl* ctl, hipStream_t s) {
+                    const SinkCtl* ctl, hipStream_t s, int reverse) {
     const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float2) * 2 * (T / kWave);
     // The kernel also holds a small static LDS queue: ask for exactly the
     // dynamic bytes this launch needs (the attribute only grows).
@@ -856,38 +880,48 @@ int launch_fused_tg(const float* C, int n, float scale2, const float* v2, float*
         return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
                     max_lds_bytes());
     if (lds > attr_bytes) {
-        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G, true>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sk_fused_sweep<T, G, false>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        CG_CHECK_HIP((set_fused_lds<T, G, true, false>(lds)));
+        CG_CHECK_HIP((set_fused_lds<T, G, false, false>(lds)));
+        CG_CHECK_HIP((set_fused_lds<T, G, true, true>(lds)));
+        CG_CHECK_HIP((set_fused_lds<T, G, false, true>(lds)));
         attr_bytes = lds;
     }
-    if (n == 4 * G * T)
-        hipLaunchKernelGGL((sk_fused_sweep<T, G, true>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
-                           p.rows_per_blk, ctl);
+    const bool keep = n <= kCacheableMaxN;
+    const int rev = keep ? (reverse & 1) : 0;
+    const bool full = n == 4 * G * T;
+    const dim3 grid(p.nblk), block(T);
+    if (full && keep)
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, true, true>), grid, block, lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, rev, ctl);
+    else if (full)
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, true, false>), grid, block, lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, rev, ctl);
+    else if (keep)
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, false, true>), grid, block, lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, rev, ctl);
     else
-        hipLaunchKernelGGL((sk_fused_sweep<T, G, false>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v2, u2, part,
-                           p.rows_per_blk, ctl);
+        hipLaunchKernelGGL((sk_fused_sweep<T, G, false, false>), grid, block, lds, s, C, n, scale2, v2, u2, part,
+                           p.rows_per_blk, rev, ctl);
     return check_launch("sk_fused_sweep");
 }
 
 int launch_fused(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
-                 const SinkCtl* ctl, hipStream_t s) {
+                 const SinkCtl* ctl, hipStream_t s, int reverse) {
     if (p.threads == 256) {
         switch (p.groups) {
-            case 1: return launch_fused_tg<256, 1>(C, n, scale2, v2, u2, part, p, ctl, s);
-            case 2: return launch_fused_tg<256, 2>(C, n, scale2, v2, u2, part, p, ctl, s);
-            case 4: return launch_fused_tg<256, 4>(C, n, scale2, v2, u2, part, p, ctl, s);
-            case 6: return launch_fused_tg<256, 6>(C, n, scale2, v2, u2, part, p, ctl, s);
-            default: return launch_fused_tg<256, 8>(C, n, scale2, v2, u2, part, p, ctl, s);
+            case 1: return launch_fused_tg<256, 1>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 2: return launch_fused_tg<256, 2>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 4: return launch_fused_tg<256, 4>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 6: return launch_fused_tg<256, 6>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            default: return launch_fused_tg<256, 8>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         }
     }
     switch (p.groups) {
-        case 1: return launch_fused_tg<1024, 1>(C, n, scale2, v2, u2, part, p, ctl, s);
-        case 2: return launch_fused_tg<1024, 2>(C, n, scale2, v2, u2, part, p, ctl, s);
-        case 4: return launch_fused_tg<1024, 4>(C, n, scale2, v2, u2, part, p, ctl, s);
-        case 6: return launch_fused_tg<1024, 6>(C, n, scale2, v2, u2, part, p, ctl, s);
-        default: return launch_fused_tg<1024, 8>(C, n, scale2, v2, u2, part, p, ctl, s);
+        case 1: return launch_fused_tg<1024, 1>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 2: return launch_fused_tg<1024, 2>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 4: return launch_fused_tg<1024, 4>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 6: return launch_fused_tg<1024, 6>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        default: return launch_fused_tg<1024, 8>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
     }
 }
 
@@ -920,12 +954,18 @@ int launch_std_fused_tg(const float* C, int n, float scale2, const float* v, flo
         return fail(CG_EUNSUPPORTED, "fused Sinkhorn sweep needs %zu bytes of LDS (device has %d)", lds,
                     max_lds_bytes());
     if (lds > attr_bytes) {
-        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G>),
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G, false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&std_fused_sweep<T, G, true>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         attr_bytes = lds;
     }
-    hipLaunchKernelGGL((std_fused_sweep<T, G>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v, u, part,
-                       p.rows_per_blk, ctl);
+    if (n <= kStdCacheableMaxN)
+        hipLaunchKernelGGL((std_fused_sweep<T, G, true>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v, u, part,
+                           p.rows_per_blk, ctl);
+    else
+        hipLaunchKernelGGL((std_fused_sweep<T, G, false>), dim3(p.nblk), dim3(T), lds, s, C, n, scale2, v, u, part,
+                           p.rows_per_blk, ctl);
     return check_launch("std_fused_sweep");
 }
 
@@ -1105,7 +1145,8 @@ int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, cons
         float* uq = u_hist + q * nn;
         float* vq = v_hist + q * nn;
         if (use_fused) {
-            if (int rc = launch_fused(C, n, scale2, vq, uq, static_cast<float*>(w.part), p, w.ctl, s)) return rc;
+            if (int rc = launch_fused(C, n, scale2, vq, uq, static_cast<float*>(w.part), p, w.ctl, s, q & 1))
+                return rc;
             if (q < steps)
                 if (int rc = comb(false, vq, vq + nn)) return rc;
         } else {
@@ -1242,7 +1283,8 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
                 const bool timed = timer_sampled(it);
                 if (timed)
                     if (int rc = timer_mark(s)) return rc;
-                if (int rc = launch_fused(C, ni, scale2, w.v2[cur], w.u2, static_cast<float*>(w.part), p, w.ctl, s))
+                if (int rc = launch_fused(C, ni, scale2, w.v2[cur], w.u2, static_cast<float*>(w.part), p, w.ctl, s,
+                                          it & 1))
                     return rc;
                 if (timed)
                     if (int rc = timer_mark(s)) return rc;
