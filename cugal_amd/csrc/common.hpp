@@ -119,19 +119,23 @@ __device__ __forceinline__ FeatureRow feature_row(const float* __restrict__ f, i
     return r;
 }
 
-// Euclidean distance between row `a` and row j of ft (the direct form; the
-// first four squares are summed in the same order for either width).
+// Euclidean distance between two feature rows (the direct form; the first
+// four squares are summed in the same order for either width).
 template <int FD>
-__device__ __forceinline__ float feature_distance(const FeatureRow& a, const float* __restrict__ ft, int64_t j) {
-    const float4 b = *reinterpret_cast<const float4*>(ft + FD * j);
-    const float d0 = a.a0.x - b.x, d1 = a.a0.y - b.y, d2 = a.a0.z - b.z, d3 = a.a0.w - b.w;
+__device__ __forceinline__ float feature_distance_rows(const FeatureRow& a, const FeatureRow& b) {
+    const float d0 = a.a0.x - b.a0.x, d1 = a.a0.y - b.a0.y, d2 = a.a0.z - b.a0.z, d3 = a.a0.w - b.a0.w;
     float s = fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3)));
     if constexpr (FD == 8) {
-        const float4 c = *reinterpret_cast<const float4*>(ft + FD * j + 4);
-        const float e0 = a.a1.x - c.x, e1 = a.a1.y - c.y, e2 = a.a1.z - c.z, e3 = a.a1.w - c.w;
+        const float e0 = a.a1.x - b.a1.x, e1 = a.a1.y - b.a1.y, e2 = a.a1.z - b.a1.z, e3 = a.a1.w - b.a1.w;
         s += fmaf(e0, e0, fmaf(e1, e1, fmaf(e2, e2, e3 * e3)));
     }
     return sqrtf(s);
+}
+
+// Distance between row `a` and row j of ft.
+template <int FD>
+__device__ __forceinline__ float feature_distance(const FeatureRow& a, const float* __restrict__ ft, int64_t j) {
+    return feature_distance_rows<FD>(a, feature_row<FD>(ft, j));
 }
 
 __host__ __device__ constexpr int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }

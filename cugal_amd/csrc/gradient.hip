@@ -32,6 +32,7 @@ struct Epilogue {
     const float* ft;  // target features n x fdim
     float rl;         // f * lambda
     int fdim;         // 4 or 8
+    int pvec;         // P 16-byte aligned with a row stride divisible by 4
 };
 
 __device__ __forceinline__ float feat_dist(float4 a, const float* __restrict__ ft, int j) {
@@ -232,6 +233,13 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
     const int g = threadIdx.x >> 4;  // 16 output rows per pass
     const int q = threadIdx.x & 15;  // four consecutive output columns
     const int rq = r0 + 4 * q;
+    // A thread's four output columns are fixed: their target feature rows
+    // are loaded once, not once per output row.
+    FeatureRow ftj[4];
+    if (EPI && ep.fs) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ftj[e] = rq + e < n ? feature_row<FD>(ep.ft, rq + e) : FeatureRow{};
+    }
     for (int cc = g; cc < TC; cc += 16) {
         const int c = c0 + cc;
         if (c >= n) break;
@@ -239,11 +247,31 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
         const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
+            // Same terms in the same order as epi_one: scale * sum, + D,
+            // + rl (1 - 2 P); P read as one 16-byte load where aligned.
             FeatureRow fsi{};
             if (ep.fs) fsi = feature_row<FD>(ep.fs, c);
+            float pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (ep.P) {
+                if (VEC && ep.pvec && rq + 3 < n) {
+                    const float4 p4 = *reinterpret_cast<const float4*>(ep.P + pbase + rq);
+                    pv[0] = p4.x;
+                    pv[1] = p4.y;
+                    pv[2] = p4.z;
+                    pv[3] = p4.w;
+                } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (rq + e < n) v[e] = epi_one<FD>(v[e], scale, ep, pbase + rq + e, fsi, rq + e);
+                    for (int e = 0; e < 4; ++e)
+                        if (rq + e < n) pv[e] = ep.P[pbase + rq + e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float gv = v[e] * scale;
+                if (ep.fs) gv += feature_distance_rows<FD>(fsi, ftj[e]);
+                if (ep.P) gv += ep.rl - 2.0f * ep.rl * pv[e];
+                v[e] = gv;
+            }
         }
         if (VEC && rq + 3 < n) {
             st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
@@ -356,7 +384,7 @@ int cg_adjacency_matmul(const int32_t* col_indices, const int32_t* row_pointers,
                         float* out, int negate, cg_stream_t stream) {
     if (!col_indices || !row_pointers || !matrix || !out || bad_n(n))
         return fail(CG_EINVAL, "cg_adjacency_matmul: bad arguments");
-    Epilogue ep{nullptr, 0, nullptr, nullptr, 0.0f, 4};
+    Epilogue ep{nullptr, 0, nullptr, nullptr, 0.0f, 4, 0};
     return launch_rowgather(col_indices, row_pointers, static_cast<int>(n), matrix, negate ? -1.0f : 1.0f, ep, out,
                             reinterpret_cast<hipStream_t>(stream));
 }
@@ -415,7 +443,7 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
     float* Pp = cv.take<float>(static_cast<size_t>(n) * ldy);
     float* Y = cv.take<float>(static_cast<size_t>(n) * ldy);
-    const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f, 4};
+    const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f, 4, 0};
     const float* X1 = P;
     int64_t ld1 = ld_p;
     if (n % 4 == 0 && aligned16(P) && !gather_friendly_ld(ld_p)) {
@@ -427,7 +455,8 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
     }
     if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
-    Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim};
+    Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim,
+                aligned16(P) && ld_p % 4 == 0 ? 1 : 0};
     return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
 }
 
