@@ -126,7 +126,7 @@ def measured_traffic(n: int):
     except (OSError, ValueError):
         return None
     kernels = data.get("kernels", {})
-    entry = kernels.get("sk_fused_sweep<1024, 8, true>") or kernels.get("sk_fused_sweep<1024, 8>")
+    entry = kernels.get("sk_fused_sweep<1024, 8, true, false>")
     if data.get("n") != n or entry is None:
         return None
     return entry["read_bytes"] + entry["write_bytes"]
