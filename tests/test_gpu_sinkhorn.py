@@ -179,6 +179,30 @@ def test_large_marginals(device, n):
     assert abs(errors[-1] - col_err) <= 1e-2 * col_err + 1e-3
 
 
+@pytest.mark.parametrize("n", [12288, 16384])
+def test_fused_matches_two_sweep_either_side_of_the_cacheable_width(device, n):
+    # The fused sweep reads matrices up to n = 12288 with caching loads and
+    # alternating row order, wider ones with non-temporal loads in a fixed
+    # order.  Both variants against the two-sweep path (pinned to the oracle
+    # at small n), from a warm start so the shifted row step runs: same
+    # potentials within the drift bound, same error trace.
+    g = torch.Generator(device=device).manual_seed(n)
+    C = torch.rand((n, n), generator=g, device=device) * 4.0
+    u0 = torch.rand(n, generator=g, device=device) - 0.5
+    v0 = torch.rand(n, generator=g, device=device) - 0.5
+    out = {}
+    for fused in (True, False):
+        u, v = u0.clone(), v0.clone()
+        errors, count, _ = solve_log_native(C, 0.3, u, v, 31, 0.0, 10, None, fused=fused)
+        assert count == 31
+        out[fused] = (u.double().cpu(), v.double().cpu(), errors)
+    (uf, vf, ef), (ut, vt, et) = out[True], out[False]
+    np.testing.assert_allclose(uf.numpy(), ut.numpy(), atol=U_ATOL / 0.3)
+    np.testing.assert_allclose(vf.numpy(), vt.numpy(), atol=U_ATOL / 0.3)
+    assert len(ef) == len(et) == 3
+    np.testing.assert_allclose(ef, et, rtol=2e-3, atol=n * 2.0 ** -24 * (4.0 + 4.0 / 0.3 * 1.4427))
+
+
 def test_reference_step_and_marginal(device):
     # cg_sinkhorn_log_step / cg_sinkhorn_log_marginal keep the reference
     # kernels' contracts (cuda_kernels/sinkhorn_log.cu:8-114).
