@@ -808,7 +808,7 @@ Plan make_plan(int64_t n, int fused) {
     if (fused) {
         p.threads = n > 8192 ? 1024 : 256;
         const int64_t g = div_up(n, static_cast<int64_t>(p.threads) * 4);
-        p.groups = g <= 1 ? 1 : g <= 2 ? 2 : g <= 4 ? 4 : g <= 6 ? 6 : 8;
+        p.groups = static_cast<int>(std::min<int64_t>(8, std::max<int64_t>(1, g)));
         // 1024-thread blocks: one per CU (256 CUs); 256-thread blocks: two.
         // The column sums of a block cost 4n bytes, 1/rpb of C.
         const int64_t target = p.threads == 1024 ? 256 : 512;
@@ -911,16 +911,22 @@ int launch_fused(const float* C, int n, float scale2, const float* v2, float* u2
         switch (p.groups) {
             case 1: return launch_fused_tg<256, 1>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
             case 2: return launch_fused_tg<256, 2>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 3: return launch_fused_tg<256, 3>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
             case 4: return launch_fused_tg<256, 4>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 5: return launch_fused_tg<256, 5>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
             case 6: return launch_fused_tg<256, 6>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+            case 7: return launch_fused_tg<256, 7>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
             default: return launch_fused_tg<256, 8>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         }
     }
     switch (p.groups) {
         case 1: return launch_fused_tg<1024, 1>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         case 2: return launch_fused_tg<1024, 2>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 3: return launch_fused_tg<1024, 3>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         case 4: return launch_fused_tg<1024, 4>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 5: return launch_fused_tg<1024, 5>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         case 6: return launch_fused_tg<1024, 6>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
+        case 7: return launch_fused_tg<1024, 7>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
         default: return launch_fused_tg<1024, 8>(C, n, scale2, v2, u2, part, p, ctl, s, reverse);
     }
 }
@@ -975,16 +981,22 @@ int launch_std_fused(const float* C, int n, float scale2, const float* v, float*
         switch (p.groups) {
             case 1: return launch_std_fused_tg<256, 1>(C, n, scale2, v, u, part, p, ctl, s);
             case 2: return launch_std_fused_tg<256, 2>(C, n, scale2, v, u, part, p, ctl, s);
+            case 3: return launch_std_fused_tg<256, 3>(C, n, scale2, v, u, part, p, ctl, s);
             case 4: return launch_std_fused_tg<256, 4>(C, n, scale2, v, u, part, p, ctl, s);
+            case 5: return launch_std_fused_tg<256, 5>(C, n, scale2, v, u, part, p, ctl, s);
             case 6: return launch_std_fused_tg<256, 6>(C, n, scale2, v, u, part, p, ctl, s);
+            case 7: return launch_std_fused_tg<256, 7>(C, n, scale2, v, u, part, p, ctl, s);
             default: return launch_std_fused_tg<256, 8>(C, n, scale2, v, u, part, p, ctl, s);
         }
     }
     switch (p.groups) {
         case 1: return launch_std_fused_tg<1024, 1>(C, n, scale2, v, u, part, p, ctl, s);
         case 2: return launch_std_fused_tg<1024, 2>(C, n, scale2, v, u, part, p, ctl, s);
+        case 3: return launch_std_fused_tg<1024, 3>(C, n, scale2, v, u, part, p, ctl, s);
         case 4: return launch_std_fused_tg<1024, 4>(C, n, scale2, v, u, part, p, ctl, s);
+        case 5: return launch_std_fused_tg<1024, 5>(C, n, scale2, v, u, part, p, ctl, s);
         case 6: return launch_std_fused_tg<1024, 6>(C, n, scale2, v, u, part, p, ctl, s);
+        case 7: return launch_std_fused_tg<1024, 7>(C, n, scale2, v, u, part, p, ctl, s);
         default: return launch_std_fused_tg<1024, 8>(C, n, scale2, v, u, part, p, ctl, s);
     }
 }

@@ -179,12 +179,15 @@ def test_large_marginals(device, n):
     assert abs(errors[-1] - col_err) <= 1e-2 * col_err + 1e-3
 
 
-@pytest.mark.parametrize("n", [12288, 16384])
-def test_fused_matches_two_sweep_either_side_of_the_cacheable_width(device, n):
-    # The fused sweep reads matrices up to n = 12288 with caching loads and
-    # alternating row order, wider ones with non-temporal loads in a fixed
-    # order.  Both variants against the two-sweep path (pinned to the oracle
-    # at small n), from a warm start so the shifted row step runs: same
+@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 16384, 20480, 28000])
+def test_fused_matches_two_sweep_at_every_block_shape(device, n):
+    # The fused sweep's block holds 1 to 8 float4 groups of a row per thread
+    # (256 threads up to n = 8192, 1024 above; 2600, 5000, 7000 take 3, 5
+    # and 7 groups with partial last groups, 12288 and 20480 exactly 3 and 5,
+    # 28000 seven), reads matrices up to n = 12288 with caching loads and
+    # alternating row order and wider ones with non-temporal loads in a
+    # fixed order.  Each against the two-sweep path (pinned to the oracle at
+    # small n), from a warm start so the shifted row step runs: same
     # potentials within the drift bound, same error trace.
     g = torch.Generator(device=device).manual_seed(n)
     C = torch.rand((n, n), generator=g, device=device) * 4.0

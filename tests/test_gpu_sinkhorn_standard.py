@@ -52,6 +52,27 @@ def std_error_floor(n):
     return n * 8 * 2.0 ** -24
 
 
+@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 20480])
+def test_fused_matches_two_sweep_at_every_block_shape(device, n):
+    # Fused STANDARD sweeps of 3, 5, 7 (partial) and 3, 5 (exact) float4
+    # groups per thread, with caching loads up to n = 8192, against the
+    # two-sweep path: same scalings, same error trace.
+    g = torch.Generator(device=device).manual_seed(n)
+    C = torch.rand((n, n), generator=g, device=device) * 5.0
+    out = {}
+    for fused in (True, False):
+        u = torch.full((n,), 1.0 / n, device=device)
+        v = torch.full((n,), 1.0 / n, device=device)
+        errors, count = solve_standard_native(C, 1.0, u, v, 31, 0.0, 10, None, fused=fused)
+        assert count == 31
+        out[fused] = (u.double().cpu(), v.double().cpu(), errors)
+    (uf, vf, ef), (ut, vt, et) = out[True], out[False]
+    np.testing.assert_allclose(uf.numpy(), ut.numpy(), rtol=S_RTOL)
+    np.testing.assert_allclose(vf.numpy(), vt.numpy(), rtol=S_RTOL)
+    assert len(ef) == len(et) == 4
+    np.testing.assert_allclose(ef, et, rtol=2e-3, atol=std_error_floor(n))
+
+
 @pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
 @pytest.mark.parametrize("fused", [True, False])
 def test_fixed_iterations_match_oracle(device, n, fused):
