@@ -53,6 +53,11 @@ class Adjacency:
     row_pointers: torch.Tensor
     """Offset in col_indices where each row begins; length n + 1 (int32)."""
 
+    loops_twice: bool = False
+    """True when a self-loop is listed twice in its row, as the reference's
+    symmetrised edge list gives it (cugal/adjacency.py:52-53): the loop
+    contributes 2 to the SpMM; the 0/1 bit form cannot hold that."""
+
     _packed_bits: torch.Tensor | None = field(default=None, init=False, repr=False, compare=False)
 
     @classmethod
@@ -65,18 +70,32 @@ class Adjacency:
         return cls(col, rp)
 
     @classmethod
-    def from_graph(cls, graph: nx.Graph, device) -> "Adjacency":
+    def from_graph(cls, graph: nx.Graph, device, loops_once: bool = False) -> "Adjacency":
         """From a networkx graph whose nodes are 0..n-1 (cugal/adjacency.py:
-        46-69).  An undirected graph gives both directions of every edge."""
+        46-69).  An undirected graph gives both directions of every edge, so
+        a self-loop is listed twice, as in the reference's CSR; with
+        `loops_once` it is listed once, the 1 on the diagonal of the
+        reference's dense adjacency (nx.to_numpy_array, cugal/pred.py:
+        106-110)."""
         dev = _device(device)
         node_count = graph.number_of_nodes()
         edge_list = list(nx.edges(graph))
         edges = np.asarray(edge_list, dtype=np.int64).reshape(-1, 2)
         if edges.size and (edges.min() < 0 or edges.max() >= node_count):
             raise ValueError("graph nodes must be labelled 0..n-1")
+        undirected = not graph.is_directed()
+        loops = int(np.count_nonzero(edges[:, 0] == edges[:, 1])) if edges.size else 0
+        if undirected and loops and loops_once:
+            # mirror the other edges on the host; the kernel then adds each
+            # pair once
+            other = edges[edges[:, 0] != edges[:, 1]]
+            edges = np.concatenate((edges, other[:, ::-1]))
+            t = torch.from_numpy(np.ascontiguousarray(edges).astype(np.int32)).to(dev)
+            col, rp = edges_to_csr(t, node_count, symmetrize=False)
+            return cls(col, rp)
         t = torch.from_numpy(edges.astype(np.int32)).to(dev)
-        col, rp = edges_to_csr(t, node_count, symmetrize=not graph.is_directed())
-        return cls(col, rp)
+        col, rp = edges_to_csr(t, node_count, symmetrize=undirected)
+        return cls(col, rp, loops_twice=undirected and loops > 0)
 
     def as_dense(self, dtype: torch.dtype) -> torch.Tensor:
         """Convert back to a dense matrix (format conversion)."""

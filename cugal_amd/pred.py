@@ -54,7 +54,16 @@ MFMA_DENSITY = 0.0077
 
 def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
     """Resolve "auto": the matrix cores when the adjacencies are dense
-    enough, else the CSR gathers."""
+    enough, else the CSR gathers.  An adjacency that lists a self-loop twice
+    (the sparse form of a graph with loops) needs the CSR gathers: the
+    matrix-core operand holds 0/1 entries."""
+    if A.loops_twice or B.loops_twice:
+        if precision == "bf16x2":
+            raise NotImplementedError(
+                "gradient_precision='bf16x2' holds 0/1 adjacency entries; with use_sparse_adjacency=True a "
+                "self-loop counts twice (the reference's CSR lists it twice): use 'fp32' or 'auto'"
+            )
+        return "fp32" if precision == "auto" else precision
     if precision != "auto":
         return precision
     n = A.number_of_nodes()
@@ -186,7 +195,7 @@ def update_quasi_permutation(P: torch.Tensor, K, u: torch.Tensor, v: torch.Tenso
     return P - before
 
 
-def _adjacency_of(graph, config: Config) -> Adjacency:
+def _adjacency_of(graph, config: Config, loops_once: bool = False) -> Adjacency:
     if isinstance(graph, Adjacency):
         return graph
     if isinstance(graph, torch.Tensor):
@@ -195,7 +204,7 @@ def _adjacency_of(graph, config: Config) -> Adjacency:
         return Adjacency.from_dense(graph.to(config.device))
     if nx.is_directed(graph):
         raise NotImplementedError("the HIP gradient handles undirected graphs")
-    return Adjacency.from_graph(graph, config.device)
+    return Adjacency.from_graph(graph, config.device, loops_once=loops_once)
 
 
 def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profile) -> torch.Tensor:
@@ -203,8 +212,10 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
     step it = 1..frank_wolfe_iter_count with alpha = 2 / (2 + it), Sinkhorn
     duals warm-started across all steps."""
     _check_config(config)
-    A = _adjacency_of(A, config)
-    B = _adjacency_of(B, config)
+    # Graphs given as networkx: the dense path's adjacency holds a self-loop
+    # once (nx.to_numpy_array), the sparse path's CSR twice.
+    A = _adjacency_of(A, config, loops_once=not config.use_sparse_adjacency)
+    B = _adjacency_of(B, config, loops_once=not config.use_sparse_adjacency)
     n = A.number_of_nodes()
     if B.number_of_nodes() != n:
         raise ValueError(f"graphs differ in size ({n} vs {B.number_of_nodes()})")
@@ -277,10 +288,14 @@ def convert_to_permutation_matrix(quasi_permutation: torch.Tensor, config: Confi
     return to_host(quasi_permutation), mapping
 
 
-def align_device(source: Adjacency, target: Adjacency, config: Config, profile: Profile | None = None):
+def align_device(source: Adjacency, target: Adjacency, config: Config, profile: Profile | None = None,
+                 gradient_adjacencies: tuple[Adjacency, Adjacency] | None = None):
     """The alignment from device-resident CSR adjacencies: features, the
     Frank-Wolfe loop and the LAP.  Returns (P on the device, column of each
-    row).  ``cugal`` is this plus graph conversion and the host copy of P."""
+    row).  ``cugal`` is this plus graph conversion and the host copy of P.
+    `gradient_adjacencies` replaces (source, target) in the gradient: the
+    dense path of a graph with self-loops (loops listed once there, twice in
+    the CSR the features are computed from)."""
     _check_config(config)
     profile = Profile() if profile is None else profile
     start_time = TimeStamp(config.device)
@@ -290,7 +305,8 @@ def align_device(source: Adjacency, target: Adjacency, config: Config, profile: 
         assert features.target.isfinite().all(), "target feature tensor has NaN values"
     profile.log_time(start_time, Phase.FEATURE_EXTRACTION)
 
-    quasi_permutation = find_quasi_permutation_matrix(source, target, features, config, profile)
+    grad_source, grad_target = gradient_adjacencies or (source, target)
+    quasi_permutation = find_quasi_permutation_matrix(grad_source, grad_target, features, config, profile)
     if config.safe_mode:
         assert quasi_permutation.isfinite().all(), "quasi_permutation tensor has NaN values"
     col_ind = hungarian(quasi_permutation, config, profile)
@@ -323,8 +339,14 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
 
     source_adj = _adjacency_of(source, config)
     target_adj = _adjacency_of(target, config)
+    gradient_adjacencies = None
+    if not config.use_sparse_adjacency and (source_adj.loops_twice or target_adj.loops_twice):
+        # The reference's dense gradient multiplies by nx.to_numpy_array (a
+        # loop is 1 on the diagonal); its features come from the CSR (twice).
+        gradient_adjacencies = (_adjacency_of(source, config, loops_once=True),
+                                _adjacency_of(target, config, loops_once=True))
 
-    quasi_permutation, col_ind = align_device(source_adj, target_adj, config, profile)
+    quasi_permutation, col_ind = align_device(source_adj, target_adj, config, profile, gradient_adjacencies)
     mapping = [(i, int(col_ind[i])) for i in range(len(quasi_permutation))]
     output = to_host(quasi_permutation), mapping
     if config.safe_mode:

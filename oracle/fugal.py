@@ -12,7 +12,10 @@ Reference quirks (see the `quirks` arguments):
     (cugal/sinkhorn.py:67-74): off by default (quirk_log_marginal=True
     reproduces it);
   * DOUBLE_GREEDY's argsort is not stable (cugal/greedy_lap.py:52): a stable
-    sort is used, identical whenever the row maxima are distinct.
+    sort is used, identical whenever the row maxima are distinct;
+  * a self-loop counts once in the dense adjacency and twice in the sparse
+    one, and the GPU path's features count its CSR entries where networkx
+    ignores it (adjacency_matrix, extract_features_csr).
 """
 
 from __future__ import annotations
@@ -52,6 +55,70 @@ def extract_features_nx(G: nx.Graph) -> np.ndarray:
         for n in node_list
     ]
     return np.nan_to_num(np.stack((degs, clusts, neighbor_degs, neighbor_clusts)).T)
+
+
+def csr_lists(G: nx.Graph) -> list[list[int]]:
+    """The sorted neighbour lists of the reference's CSR for an undirected
+    graph labelled 0..n-1: every edge of nx.edges(G) in both directions, so
+    a self-loop is listed twice (cugal/adjacency.py:46-69)."""
+    n = G.number_of_nodes()
+    rows = [[] for _ in range(n)]
+    for x, y in G.edges():
+        rows[x].append(y)
+        rows[y].append(x)
+    return [sorted(r) for r in rows]
+
+
+def extract_features_csr(G: nx.Graph) -> np.ndarray:
+    """The four features as the reference's GPU path computes them
+    (cugal/feature_extraction.py:15-38 over cuda_kernels/
+    feature_extraction.cu:13-106): degree = CSR row length; t = sum over
+    the row's entries k of the sorted-merge intersection size of rows v and
+    k; clustering = t / (d (d - 1)), 0 when t = 0 or d < 2; neighbour means
+    over the row's entries, 0 for an empty row.  On simple graphs this is
+    extract_features_nx; with self-loops the two reference paths differ
+    (networkx ignores a loop for clustering and the means).
+    Pin: equal to extract_features_nx on simple graphs; on graphs with loops
+    the multiset formula t(v) = sum_k M[v,k] sum_w min(M[v,w], M[k,w]) over
+    the entry-count matrix M, and closed forms (a lone looped vertex: 2, 2,
+    2, 2)."""
+    rows = csr_lists(G)
+    n = len(rows)
+
+    def merge_count(a, b):
+        i = j = c = 0
+        while i < len(a) and j < len(b):
+            if a[i] == b[j]:
+                c += 1
+                i += 1
+                j += 1
+            elif a[i] > b[j]:
+                j += 1
+            else:
+                i += 1
+        return c
+
+    deg = np.array([len(r) for r in rows], dtype=np.float64)
+    t = np.array([sum(merge_count(rows[v], rows[k]) for k in rows[v]) for v in range(n)], dtype=np.float64)
+    clus = np.where((t == 0) | (deg < 2), 0.0, t / np.maximum(deg * (deg - 1), 1.0))
+    nd = np.array([np.mean([deg[k] for k in r]) if r else 0.0 for r in rows])
+    nc = np.array([np.mean([clus[k] for k in r]) if r else 0.0 for r in rows])
+    return np.stack((deg, clus, nd, nc)).T
+
+
+def adjacency_matrix(G: nx.Graph, sparse: bool, dtype=torch.float64) -> torch.Tensor:
+    """The adjacency the reference's gradient multiplies by: the dense path
+    takes nx.to_numpy_array (a self-loop is 1 on the diagonal,
+    cugal/pred.py:106-110); the sparse path its CSR, where a loop is listed
+    twice (cugal/adjacency.py:52-53), so the diagonal holds 2.
+    Pin: equal to the scipy CSR of the graph on simple graphs; the loop
+    diagonal by construction of csr_lists."""
+    A = torch.tensor(nx.to_numpy_array(G, nodelist=sorted(G.nodes())), dtype=dtype)
+    if sparse:
+        for x, y in G.edges():
+            if x == y:
+                A[x, x] = 2.0
+    return A
 
 
 def triangle_counts(adjacency: np.ndarray) -> np.ndarray:
@@ -373,12 +440,20 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
     Pin: exact recovery of the permutation on noise-free isomorphic pairs
     without automorphisms."""
     pad_graphs(source, target)
-    extract = extract_features_nx if config.node_features == "cugal" else extract_features_fugal
+
+    def extract(G):
+        if config.node_features != "cugal":
+            return extract_features_fugal(G)
+        # Graphs with self-loops: the features of the reference's GPU path
+        # (the path a device run stands in for); the CPU path's networkx
+        # features differ there and agree everywhere else.
+        return extract_features_csr(G) if nx.number_of_selfloops(G) else extract_features_nx(G)
+
     fs = torch.tensor(extract(source), dtype=dtype) * config.mu
     ft = torch.tensor(extract(target), dtype=dtype) * config.mu
     D = distance_matrix(fs, ft)
-    A = torch.tensor(nx.to_numpy_array(source, nodelist=sorted(source.nodes())), dtype=dtype)
-    B = torch.tensor(nx.to_numpy_array(target, nodelist=sorted(target.nodes())), dtype=dtype)
+    A = adjacency_matrix(source, config.use_sparse_adjacency, dtype)
+    B = adjacency_matrix(target, config.use_sparse_adjacency, dtype)
     n = A.shape[0]
     state = SinkhornState(n, config.sinkhorn_method, dtype)
     P = torch.full((n, n), 1 / n, dtype=dtype)

@@ -187,3 +187,30 @@ def test_triangles_large_graph_invariant(device):
     adj = Adjacency.from_graph(g, device)
     tri = triangle_counts(adj).cpu().numpy().astype(np.int64)
     assert tri.sum() == 6 * (sum(nx.triangles(g).values()) // 3)
+
+
+def test_features_with_self_loops_follow_the_gpu_path(device):
+    # Self-loops (one of the reference's datasets, in-arenas, has one): the
+    # CSR lists a loop twice, as the reference's symmetrised edge list does,
+    # and the features count CSR entries as the reference's GPU kernels do
+    # (its networkx path ignores a loop for clustering and the means).
+    # Node 40 has only a loop.
+    g = nx.karate_club_graph()
+    g.add_edges_from([(0, 0), (5, 5), (33, 33), (40, 40)])
+    g = _with_all_nodes(g)
+    adj = Adjacency.from_graph(g, device)
+    assert adj.loops_twice
+    rows = adj.row_pointers.cpu().numpy()
+    cols = adj.col_indices.cpu().numpy()
+    assert list(cols[rows[40]:rows[41]]) == [40, 40]
+    once = Adjacency.from_graph(g, device, loops_once=True)
+    assert not once.loops_twice
+    assert once.col_indices.numel() == adj.col_indices.numel() - 4
+    want_once = scipy.sparse.csr_matrix(nx.to_scipy_sparse_array(g, nodelist=range(g.number_of_nodes()), format="csr"))
+    want_once.sort_indices()
+    assert np.array_equal(once.row_pointers.cpu().numpy(), want_once.indptr)
+    assert np.array_equal(once.col_indices.cpu().numpy(), want_once.indices)
+    got = node_features(adj, 1.0).cpu().numpy()
+    want = oracle.extract_features_csr(g)
+    assert np.array_equal(got[:, 0], want[:, 0].astype(np.float32))
+    np.testing.assert_allclose(got[:, 1:], want[:, 1:].astype(np.float32), rtol=2 ** -23, atol=0)

@@ -214,3 +214,44 @@ def test_tiny_graphs_match_oracle(device, name, method, lap):
         assert abs(R[rows, cols].sum() - R[rows, ref_cols].sum()) <= 1e-6 * len(cols)
     else:
         assert cols == ref_cols
+
+
+@pytest.mark.parametrize("method,sparse,precision", [("LOG", True, "auto"), ("LOG", False, "fp32"),
+                                                     ("STANDARD", True, "fp32"), ("STANDARD", False, "fp32"),
+                                                     ("LOG", False, "bf16x2")])
+def test_self_loops_match_oracle(device, method, sparse, precision):
+    # A looped graph (G(40, 0.12), seed 3, no automorphism, with loops at 0,
+    # 5, 33 and a lone looped node 40) against its relabelled copy: the
+    # sparse path counts a loop twice (the reference's CSR), the dense path
+    # once (nx.to_numpy_array); features are the GPU path's in both.  Fixed
+    # iteration counts, so Q follows the FP64 oracle entry by entry.
+    g = nx.Graph(nx.erdos_renyi_graph(40, 0.12, seed=3))
+    g.add_edges_from([(0, 0), (5, 5), (33, 33), (40, 40)])
+    g.add_nodes_from(range(41))
+    perm = np.random.default_rng(3).permutation(41)
+    h = nx.relabel_nodes(g, {i: int(perm[i]) for i in range(41)})
+    reg = 0.3 if method == "LOG" else 1.0
+    gpu = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod[method],
+                 sinkhorn_regularization=reg, sinkhorn_threshold=0.0, sinkhorn_iterations=40, mu=1.0, iter_count=3,
+                 frank_wolfe_iter_count=3, use_sparse_adjacency=sparse, gradient_precision=precision,
+                 hungarian_method=HungarianMethod.SCIPY)
+    ref = oracle.OracleConfig(sinkhorn_regularization=reg, sinkhorn_method=method, sinkhorn_threshold=0.0, mu=1.0,
+                              iter_count=3, frank_wolfe_iter_count=3, use_sparse_adjacency=sparse,
+                              hungarian_method="SCIPY")
+    ref.sinkhorn_iterations = 40
+    P, mapping = cugal(copy.deepcopy(g), copy.deepcopy(h), gpu, Profile())
+    run = oracle.align(copy.deepcopy(g), copy.deepcopy(h), ref)
+    R = run.P.numpy()
+    # bf16x2: the hi + lo split of P carries 16 significant bits
+    rtol = 1e-5 if precision != "bf16x2" else 1e-4
+    np.testing.assert_allclose(P, R, rtol=rtol, atol=1e-6 * float(R.max()))
+    assert [m for _, m in mapping] == [m for _, m in run.mapping]
+
+
+def test_self_loops_refuse_the_matrix_core_form_on_the_sparse_path(device):
+    g = nx.Graph(nx.karate_club_graph())
+    g.add_edge(0, 0)
+    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG, iter_count=1,
+                 frank_wolfe_iter_count=1, use_sparse_adjacency=True, gradient_precision="bf16x2")
+    with pytest.raises(NotImplementedError, match="self-loop counts twice"):
+        cugal(copy.deepcopy(g), copy.deepcopy(g), cfg, Profile())

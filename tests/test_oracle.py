@@ -66,6 +66,58 @@ def test_triangles_and_clustering_brute_force(g):
     np.testing.assert_allclose(f[:, 3], nc, rtol=1e-14, atol=1e-15)
 
 
+@pytest.mark.parametrize("g", [nx.karate_club_graph(), nx.petersen_graph(), nx.star_graph(7),
+                               nx.erdos_renyi_graph(60, 0.1, seed=3), nx.watts_strogatz_graph(30, 6, 0.3, seed=2)])
+def test_gpu_path_features_equal_networkx_on_simple_graphs(g):
+    g = nx.Graph(g)
+    g.add_nodes_from(range(max(g.nodes()) + 2))  # one isolated node
+    np.testing.assert_allclose(oracle.extract_features_csr(g), oracle.extract_features_nx(g), rtol=1e-14, atol=0)
+
+
+def _loop_graph():
+    g = nx.Graph(nx.karate_club_graph())
+    g.add_edges_from([(0, 0), (5, 5), (33, 33), (40, 40)])
+    g.add_nodes_from(range(41))
+    return g
+
+
+def test_gpu_path_features_with_self_loops():
+    # Entry-count matrix M (a loop is two entries of its row); the sorted
+    # merge of two rows counts min(M[v,w], M[k,w]) matches per w, and row v
+    # visits k M[v,k] times.
+    g = _loop_graph()
+    n = g.number_of_nodes()
+    M = np.zeros((n, n), dtype=np.int64)
+    for x, y in g.edges():
+        M[x, y] += 1
+        M[y, x] += 1
+    t = np.array([sum(M[v, k] * np.minimum(M[v], M[k]).sum() for k in range(n)) for v in range(n)])
+    d = M.sum(1).astype(np.float64)
+    clus = np.where((t == 0) | (d < 2), 0.0, t / np.maximum(d * (d - 1), 1))
+    nd = np.where(d > 0, (M @ d) / np.maximum(d, 1), 0.0)
+    nc = np.where(d > 0, (M @ clus) / np.maximum(d, 1), 0.0)
+    f = oracle.extract_features_csr(g)
+    np.testing.assert_array_equal(f[:, 0], d)
+    np.testing.assert_allclose(f[:, 1], clus, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(f[:, 2], nd, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(f[:, 3], nc, rtol=1e-14, atol=1e-15)
+    # a lone looped vertex: row [v, v], t = 2 + 2, clustering 4 / (2 * 1)
+    np.testing.assert_array_equal(f[40], [2.0, 2.0, 2.0, 2.0])
+    # and the CPU path disagrees there: networkx ignores the loop
+    np.testing.assert_array_equal(oracle.extract_features_nx(g)[40], [2.0, 0.0, 0.0, 0.0])
+
+
+def test_adjacency_counts_a_loop_once_dense_twice_sparse():
+    g = _loop_graph()
+    dense = oracle.adjacency_matrix(g, sparse=False).numpy()
+    sparse = oracle.adjacency_matrix(g, sparse=True).numpy()
+    off = ~np.eye(g.number_of_nodes(), dtype=bool)
+    np.testing.assert_array_equal(dense[off], sparse[off])
+    for v in (0, 5, 33, 40):
+        assert dense[v, v] == 1.0 and sparse[v, v] == 2.0
+    assert np.count_nonzero(np.diag(sparse)) == 4
+
+
 def test_karate_and_petersen_totals():
     assert oracle.triangle_counts(_adj(nx.karate_club_graph())).sum() == 6 * 45
     assert oracle.triangle_counts(_adj(nx.petersen_graph())).sum() == 0
