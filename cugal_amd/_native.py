@@ -129,7 +129,18 @@ def query(name: str, *args):
 
 
 def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
-    return None if t is None else t.data_ptr()
+    """Device address of a tensor.  torch reports 0 for a tensor with no
+    elements; a zero-element view of an allocated buffer (the column array of
+    a graph without edges) passes the buffer's address instead, so the entry
+    points' null checks accept it (nothing is read through it)."""
+    if t is None:
+        return None
+    p = t.data_ptr()
+    if p == 0 and t.numel() == 0:
+        storage = t.untyped_storage()
+        if storage.nbytes() > 0:
+            return storage.data_ptr() + t.storage_offset() * t.element_size()
+    return p
 
 
 def row_stride(t: torch.Tensor, name: str) -> int:
