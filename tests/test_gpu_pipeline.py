@@ -173,6 +173,8 @@ _TINY = {
     "edge": (lambda: nx.Graph([(0, 1)]), [1, 0]),
     # a tree with branches of lengths 1, 2 and 3 at node 2: no automorphism
     "asymmetric_tree": (lambda: nx.Graph([(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6)]), [4, 0, 6, 2, 5, 3, 1]),
+    # one node: every kernel at n = 1
+    "single_node": (lambda: nx.empty_graph(1), [0]),
     # no edges: zero features, P stays uniform, every LAP breaks exact ties
     "edgeless": (lambda: nx.empty_graph(5), [3, 0, 4, 1, 2]),
     # the same tree on labels 0-2, 4-7: cugal() pads the source with node 3
