@@ -65,6 +65,14 @@ def test_grad_fused_and_separate_passes(device, n, k, T, offset):
     _grad_vs_autograd(device, C, 0.4, T, k, offset)
 
 
+@pytest.mark.parametrize("n", [2, 3, 4, 8, 17, 333])
+def test_grad_small_and_odd_sizes(device, n):
+    # Widths far below a block (4 and 8 take the fused adjoint sweep with
+    # one live thread group; the odd ones the separate passes).
+    C = _rand((n, n), 51 + n, 3.0).float()
+    _grad_vs_autograd(device, C, 0.5, 7, 3)
+
+
 def test_grad_longest_segment(device):
     # 64 iterations per segment: the gradient pass holds (4 * 64 + 1) * 64
     # floats of LDS (above the 64 KiB a launch gets without asking).
