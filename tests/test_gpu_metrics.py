@@ -117,3 +117,54 @@ def test_experiment_runs_end_to_end(device, tmp_path):
     res.dump(str(tmp_path))
     back = ExperimentResults.from_dict(json.load(open(tmp_path / (res.name() + ".json"))))
     assert back.to_dict() == res.to_dict()
+
+
+def test_harness_on_a_reference_dataset_matches_oracle(device):
+    # ca-netscience (379 nodes, 914 edges; the reference's benchmark/data
+    # file, copied to tests/golden) with the configuration and the first
+    # noise level of the reference's published run (CUGAL_res.json: LOG,
+    # reg 0.3, <= 500 iterations, thr 1e-3, mu 2, 15 x 10 FW, sparse, SCIPY;
+    # 10 % of source edges removed; seed 283841238).  The harness on the
+    # device against the FP64 oracle pipeline on the same seeded instance,
+    # within 2 pp of node accuracy and of each structure score, not 0.5:
+    # this instance is ill-conditioned for the oracle itself.  torch's CPU
+    # reductions split by thread count, and the FP64 oracle moves with them:
+    # 1 or 8 threads 30163 Sinkhorn iterations and accuracy 0.6201, 16
+    # threads 26879 and 0.6095 on one machine, 0.6016 on another (threshold
+    # crossings shift, and many structurally equivalent nodes have nearly
+    # equal rows of Q that the LAP orders by rounding).  The device gave
+    # 0.6069.  (The published 0.642 averages four instances drawn after
+    # three other graphs, from a build with the LOG marginal-check quirk; it
+    # is context, not a bound.)
+    import copy
+    import os
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    seed = 283841238
+    level = NoiseLevel(0.1, 0.0, False)
+    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                 sinkhorn_regularization=0.3, sinkhorn_iterations=500, sinkhorn_threshold=1e-3, mu=2.0,
+                 iter_count=15, frank_wolfe_iter_count=10, use_sparse_adjacency=True,
+                 hungarian_method=HungarianMethod.SCIPY)
+    exp = Experiment(algorithms=[Algorithm(cfg, False)], graphs=[Graph(GraphKind.CA_NETSCIENCE, {})],
+                     noise_levels=[level], seed=seed, num_runs=1, data_dir=golden)
+    result = exp.run().results[0][0][0]
+
+    gen = np.random.default_rng(seed=seed)
+    s, t = Graph(GraphKind.CA_NETSCIENCE, {}).get(gen, golden)
+    assert (s.number_of_nodes(), s.number_of_edges()) == (379, 914)
+    source, target, mapping, _ = generate_graph(s, t, gen, level)
+    ref = oracle.OracleConfig(sinkhorn_regularization=0.3, sinkhorn_method="LOG", sinkhorn_threshold=1e-3, mu=2.0,
+                              iter_count=15, frank_wolfe_iter_count=10, use_sparse_adjacency=True,
+                              hungarian_method="SCIPY")
+    run = oracle.align(copy.deepcopy(source), copy.deepcopy(target), ref)
+    answer = np.array([m for _, m in run.mapping])
+    ref_acc = float(np.mean(answer == np.asarray(mapping)))
+    ref_scores = _oracle_scores(source, target, answer, mapping)
+    print(f"\nca-netscience, 10 % source noise: accuracy {result.accuracy:.4f} (device harness) vs "
+          f"{ref_acc:.4f} (FP64 oracle); ICS/EC/S3 {result.ics:.4f}/{result.ec:.4f}/{result.sss:.4f} vs "
+          f"{ref_scores[0]:.4f}/{ref_scores[1]:.4f}/{ref_scores[2]:.4f}")
+    assert abs(result.accuracy - ref_acc) <= 0.02
+    for got, want in zip((result.ics, result.ec, result.sss), ref_scores):
+        assert abs(got - want) <= 0.02
+    assert result.profile.frank_wolfe_iterations == 150
