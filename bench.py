@@ -25,7 +25,9 @@ matrix-core rate, and the gradient's error against FP64 on sampled rows),
 and so does the differentiation form (`differentiation_form`: Erdos-Renyi
 n=4096, 100 unrolled log-Sinkhorn iterations with checkpoints every 10;
 forward and hand-written backward times and the gradients' error against
-torch autograd in FP64).
+torch autograd in FP64).  `reference_case` is the reference's own
+CPU-runnable case (Erdos-Renyi n=1000 with its default Config), cugal() on
+the device and the FP64 oracle on the host cores, each timed end to end.
 
 Other workloads (`--workload`): `ba8192` (Barabasi-Albert n=8192, m=5, 2%
 edge noise; the LAP as greedy and as the exact device Hungarian, checked
@@ -279,6 +281,45 @@ def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: in
     }
 
 
+def reference_case(seed: int = 0):
+    """The reference's own CPU-runnable case: Erdos-Renyi G(1000, 0.01),
+    seed 0, 5 % of edges removed and as many non-edges added, seeded
+    permutation; the reference's default Config (cugal/config.py:59-120:
+    STANDARD, reg 1.0, <= 500 iterations, thr 1e-3, mu 0.5, 15 x 10
+    Frank-Wolfe steps, dense adjacency with the doubled lambda term,
+    DOUBLE_GREEDY).  cugal() on the device in FP32 and the FP64 oracle
+    pipeline on the host cores, each timed end to end (graphs in, mapping
+    out), with both node accuracies."""
+    import torch
+
+    from cugal_amd import Config, cugal
+    from cugal_amd.profile import Profile
+    from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi
+    from oracle import fugal as oracle
+
+    inst = erdos_renyi(1000, 0.01, 0.05, seed=seed)
+    cfg = Config(device="cuda:0", dtype=torch.float32)
+    cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), cfg, Profile())  # warm-up
+    torch.cuda.synchronize()
+    profile = Profile()
+    t0 = time.perf_counter()
+    _, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), cfg, profile)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), oracle.OracleConfig())
+    cpu_s = time.perf_counter() - t0
+    return {
+        "workload": "erdos_renyi n=1000 p=0.01 seed 0, 5% edge noise + refill, seeded permutation; the reference's "
+                    "default Config (STANDARD, reg 1.0, <=500 its, thr 1e-3, mu 0.5, 15x10 FW, dense, DOUBLE_GREEDY)",
+        "gpu_seconds": gpu_s, "gpu_accuracy": alignment_accuracy(inst.source_mapping, answer_columns(mapping)),
+        "gpu_sinkhorn_iterations": sum(p.iteration_count for p in profile.sinkhorn_profiles),
+        "cpu_seconds": cpu_s, "cpu_accuracy": alignment_accuracy(inst.source_mapping, answer_columns(run.mapping)),
+        "cpu_sinkhorn_iterations": sum(run.sinkhorn_iterations), "cpu_threads": torch.get_num_threads(),
+        "cpu": "FP64 oracle (restatement of the reference's CPU path), end to end on the host cores",
+    }
+
+
 def probe(args) -> int:
     """A few launches of every hot kernel on the bench workload, plus a
     device copy of one n x n FP32 matrix (4 n^2 bytes read, 4 n^2 written)
@@ -470,7 +511,12 @@ def main() -> int:
     }
 
     cpu = None
+    ref_case = None
     if not args.no_cpu:
+        ref_case = reference_case()
+        log(f"reference case ER(1000), default Config: device {ref_case['gpu_seconds']:.2f}s acc "
+            f"{ref_case['gpu_accuracy']:.4f}; FP64 oracle on {ref_case['cpu_threads']} threads "
+            f"{ref_case['cpu_seconds']:.2f}s acc {ref_case['cpu_accuracy']:.4f}")
         log("timing the FP64 CPU oracle sample")
         cpu = cpu_baseline(min(n, 16384), args.reg, args.cpu_seconds, iters_total)
         if n > 16384:
@@ -492,7 +538,7 @@ def main() -> int:
         "sinkhorn_gbs": achieved, "phase_times": phase, "api_extra_s": api_extra, "plain": plain,
         "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
         "precision_form": precision, "differentiation_form": diff,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "reference_case": ref_case,
     }
     print(json.dumps(out), flush=True)
     return 0
