@@ -31,7 +31,8 @@ the device and the FP64 oracle on the host cores, each timed end to end.
 
 Other workloads (`--workload`): `ba8192` (Barabasi-Albert n=8192, m=5, 2%
 edge noise; the LAP as greedy and as the exact device Hungarian, checked
-against scipy on the same P) and `sbm65536` (stochastic block model, 64
+against scipy on the same P; the FP64 oracle timed per stage and
+extrapolated) and `sbm65536` (stochastic block model, 64
 blocks of 1024, intra/inter degrees 48/16, 1% noise, 15 lambda stages of one
 Frank-Wolfe step, greedy LAP; P is 17 GB) and `er4096grad` (the
 differentiation form alone).
@@ -278,6 +279,39 @@ def cpu_baseline(n: int, reg: float, seconds_budget: float, total_iterations: in
         "kind": "port",
         "sample": f"FP64 oracle log-Sinkhorn at n={n}: {done} iteration(s) timed ({per_iter:.2f} s each), "
                   f"scaled to the {total_iterations} iterations of the GPU alignment",
+    }
+
+
+def cpu_stage_baseline(source, target, reg: float, fw_steps: int, iterations: int, seconds_budget: float):
+    """Per-stage FP64 oracle timing at the workload's size, extrapolated to
+    the GPU run: one sparse-form gradient (cugal/pred.py:54-74, dense torch
+    products on the host) times the Frank-Wolfe step count, plus the
+    log-Sinkhorn iteration time (cpu_baseline) times the GPU run's
+    iteration count."""
+    import networkx as nx
+    import torch
+
+    from oracle import fugal as oracle
+
+    n = source.number_of_nodes()
+    A = torch.tensor(nx.to_numpy_array(source, nodelist=range(n)), dtype=torch.float64)
+    B = torch.tensor(nx.to_numpy_array(target, nodelist=range(n)), dtype=torch.float64)
+    g = torch.Generator().manual_seed(0)
+    P = torch.rand((n, n), generator=g, dtype=torch.float64)
+    P /= P.sum(1, keepdim=True)
+    D = torch.rand((n, n), generator=g, dtype=torch.float64)
+    t0 = time.perf_counter()
+    oracle.sparse_gradient(A, B, P, D, 3)
+    t_grad = time.perf_counter() - t0
+    del A, B, P, D
+    sk = cpu_baseline(n, reg, seconds_budget, iterations)
+    return {
+        "value": t_grad * fw_steps + sk["value"],
+        "unit": "s (gradient + Sinkhorn, extrapolated per stage)",
+        "cores": torch.get_num_threads(),
+        "kind": "port",
+        "sample": f"FP64 oracle at n={n}: one gradient {t_grad:.2f} s x {fw_steps} Frank-Wolfe steps; "
+                  + sk["sample"],
     }
 
 
@@ -611,6 +645,11 @@ def run_ba8192(args) -> int:
     }
     log(f"ba8192: {e:.2f}s acc {acc:.4f}; hungarian {t_dense * 1e3:.1f} ms vs scipy {t_scipy * 1e3:.0f} ms, "
         f"equal={extra['lap']['hungarian_equals_scipy']}")
+    if not args.no_cpu:
+        log("timing the FP64 CPU oracle per stage")
+        extra["cpu_baseline"] = cpu_stage_baseline(inst.source, inst.target, args.reg, prof.frank_wolfe_iterations,
+                                                   iters, args.cpu_seconds)
+        log(f"FP64 oracle estimate: {extra['cpu_baseline']['value']:.0f}s ({extra['cpu_baseline']['sample']})")
     print(_contract_line(e, args, f"barabasi_albert n={n} m=5, 2% edge noise, seeded permutation; LOG reg "
                                   f"{args.reg}, {args.iter_count}x{args.fw_iter_count} FW, DENSE (device Hungarian)",
                          n, extra), flush=True)
