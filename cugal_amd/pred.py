@@ -334,6 +334,12 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
     node_count = max(max(source.nodes()), max(target.nodes()))
     source.add_nodes_from(set(range(node_count)) - set(source.nodes()))
     target.add_nodes_from(set(range(node_count)) - set(target.nodes()))
+    if source.number_of_nodes() != target.number_of_nodes():
+        # range(max label) leaves out the largest label, so one graph can
+        # keep a node the other lacks (the reference then fails on a shape
+        # mismatch inside the gradient)
+        raise ValueError(f"graphs differ in size after padding ({source.number_of_nodes()} vs "
+                         f"{target.number_of_nodes()} nodes)")
     if config.node_features == "fugal7" and (nx.number_of_selfloops(source) or nx.number_of_selfloops(target)):
         raise ValueError("the 7-feature set is defined here for simple graphs (no self-loops)")
 

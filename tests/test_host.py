@@ -97,6 +97,21 @@ def test_cpu_device_is_rejected():
         cugal(nx.path_graph(4), nx.path_graph(4), Config())
 
 
+def test_input_errors_raise_before_device_work():
+    # Both checks run on the host before anything touches a device, so they
+    # raise here without a GPU.
+    import torch
+
+    from cugal_amd import cugal
+
+    cfg = Config(device="cuda:0", dtype=torch.float32)
+    # the padding adds range(max label): label 4 exists only in the source
+    with pytest.raises(ValueError, match="differ in size"):
+        cugal(nx.path_graph(5), nx.path_graph(3), cfg)
+    with pytest.raises(NotImplementedError, match="undirected"):
+        cugal(nx.DiGraph(nx.path_graph(4)), nx.DiGraph(nx.path_graph(4)), cfg)
+
+
 def test_synthetic_noise_free_target_is_permuted_source():
     from cugal_amd.synthetic import erdos_renyi
 
