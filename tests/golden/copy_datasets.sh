@@ -5,3 +5,4 @@
 set -e
 cd "$(dirname "$0")"
 cp /root/reference/benchmark/data/ca-netscience.txt ca-netscience.txt
+cp /root/reference/benchmark/data/in-arenas.txt in-arenas.txt
