@@ -60,7 +60,9 @@ def extract_features_nx(G: nx.Graph) -> np.ndarray:
 def csr_lists(G: nx.Graph) -> list[list[int]]:
     """The sorted neighbour lists of the reference's CSR for an undirected
     graph labelled 0..n-1: every edge of nx.edges(G) in both directions, so
-    a self-loop is listed twice (cugal/adjacency.py:46-69)."""
+    a self-loop is listed twice (cugal/adjacency.py:46-69).
+    Pin: through extract_features_csr (equal to networkx on simple graphs;
+    the entry-count matrix built from G.edges() on looped graphs)."""
     n = G.number_of_nodes()
     rows = [[] for _ in range(n)]
     for x, y in G.edges():
