@@ -81,6 +81,10 @@ def _loop_graph():
     return g
 
 
+# networkx's path takes the mean of an empty list for the lone looped node
+# and maps the NaN to 0, as the reference does
+@pytest.mark.filterwarnings("ignore:Mean of empty slice:RuntimeWarning")
+@pytest.mark.filterwarnings("ignore:invalid value encountered:RuntimeWarning")
 def test_gpu_path_features_with_self_loops():
     # Entry-count matrix M (a loop is two entries of its row); the sorted
     # merge of two rows counts min(M[v,w], M[k,w]) matches per w, and row v
