@@ -124,7 +124,7 @@ class Config:
     """Arithmetic of the A P B contraction: "fp32" (CSR adjacency times
     dense FP32 P), "bf16x2" (the 0/1 adjacency, exact in BF16, times P split
     into BF16 hi + lo on the matrix cores, FP32 accumulation) or "auto" (the
-    matrix cores when the adjacencies hold more than ~0.77 % of n^2
+    matrix cores when the adjacencies hold more than ~1.4 % of n^2
     non-zeros, where they become the faster form, else "fp32")."""
 
     node_features: str = "cugal"

@@ -47,9 +47,12 @@ def _check_config(config: Config) -> None:
 
 
 # Density above which the matrix-core gradient beats the CSR gathers on one
-# MI355X (measured at n = 32768: the gathers cost ~7.4e-13 s per nnz * n,
-# the two GEMMs ~5.7e-15 s per n^3, so they cross at nnz ~ 0.0077 n^2).
-MFMA_DENSITY = 0.0077
+# MI355X.  Measured per gradient on Erdos-Renyi pairs at n = 16384: CSR
+# 7.1 / 13.8 / 21.4 / 28.4 / 39.9 ms at densities 0.4 / 0.8 / 1.2 / 1.6 /
+# 2.0 %, matrix cores 28.1-28.4 ms throughout (crossing at 1.6 %); at
+# n = 32768 the gathers cost ~4.1e-13 s per nnz * n above a 2.7 ms floor
+# and the GEMMs 205 ms, crossing near 1.3 %.
+MFMA_DENSITY = 0.014
 
 
 def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
