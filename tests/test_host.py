@@ -97,6 +97,19 @@ def test_cpu_device_is_rejected():
         cugal(nx.path_graph(4), nx.path_graph(4), Config())
 
 
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    # Without the HIP library every entry point raises; nothing falls back
+    # to torch or the host.
+    from cugal_amd import _native as nat
+
+    monkeypatch.setattr(nat, "LIB_PATH", str(tmp_path / "libcugal_hip.so"))
+    monkeypatch.setattr(nat, "_lib", None)
+    with pytest.raises(nat.NativeError, match="is not built"):
+        nat.lib()
+    with pytest.raises(nat.NativeError, match="is not built"):
+        nat.query("cg_sinkhorn_workspace", 64, 1, 10, 10)
+
+
 def test_input_errors_raise_before_device_work():
     # Both checks run on the host before anything touches a device, so they
     # raise here without a GPU.
