@@ -74,14 +74,47 @@ class Instance:
     description: dict
 
 
-def make_instance(graph: nx.Graph, noise: NoiseLevel, seed: int = 0, permutation_seed: int = 0,
-                  description: dict | None = None) -> Instance:
+@dataclass
+class EdgeInstance:
+    """An instance kept as edge arrays: (E, 2) int64 pairs over nodes
+    0..n-1.  Large instances skip networkx (building a 2-million-edge graph
+    object costs tens of seconds); ``adjacencies`` builds the CSR of both
+    graphs on the device straight from the pairs, the same CSR as
+    ``Adjacency.from_graph`` of the padded graphs."""
+
+    source_edges: np.ndarray
+    target_edges: np.ndarray
+    n: int
+    source_mapping: np.ndarray
+    description: dict
+
+    def graphs(self) -> Instance:
+        source = nx.from_edgelist(self.source_edges.tolist())
+        target = nx.from_edgelist(self.target_edges.tolist())
+        return Instance(source, target, self.source_mapping, dict(self.description))
+
+    def adjacencies(self, device):
+        import torch
+
+        from .adjacency import Adjacency, edges_to_csr
+
+        out = []
+        for edges in (self.source_edges, self.target_edges):
+            t = torch.from_numpy(np.ascontiguousarray(edges, dtype=np.int32)).to(device)
+            col, rp = edges_to_csr(t, self.n, symmetrize=True)
+            out.append(Adjacency(col, rp))
+        return tuple(out)
+
+
+def make_edge_instance(source_edges: np.ndarray, noise: NoiseLevel, seed: int = 0, permutation_seed: int = 0,
+                       description: dict | None = None) -> EdgeInstance:
     """generate_graph + add_synthetic_noise (benchmark/experiment.py:81-112,
-    :315-324).  The generator draws in the reference's order: source
-    removal, target removal, source refill, target refill.  The draws of
-    remove_edges are taken as one vector per call, the same stream values."""
+    :315-324) on edge arrays.  The generator draws in the reference's order:
+    source removal, target removal, source refill, target refill.  The draws
+    of remove_edges are taken as one vector per call, the same stream
+    values."""
     generator = np.random.default_rng(seed)
-    source_edges = np.array(graph.edges)
+    source_edges = np.asarray(source_edges, dtype=np.int64)
     if np.amin(source_edges) != 0:
         source_edges = source_edges - np.amin(source_edges)
     dimension = int(np.amax(source_edges)) + 1
@@ -92,9 +125,14 @@ def make_instance(graph: nx.Graph, noise: NoiseLevel, seed: int = 0, permutation
     if noise.refill_edges:
         source_edges = refill_edges(source_edges, dimension, edge_count - source_edges.shape[0], generator)
         target_edges = refill_edges(target_edges, dimension, edge_count - target_edges.shape[0], generator)
-    source = nx.from_edgelist(source_edges.tolist())
-    target = nx.from_edgelist(target_edges.tolist())
-    return Instance(source, target, source_mapping, dict(description or {}))
+    return EdgeInstance(source_edges, target_edges, dimension, source_mapping, dict(description or {}))
+
+
+def make_instance(graph: nx.Graph, noise: NoiseLevel, seed: int = 0, permutation_seed: int = 0,
+                  description: dict | None = None) -> Instance:
+    """make_edge_instance on the edges of a networkx graph, returned as
+    networkx graphs."""
+    return make_edge_instance(np.array(graph.edges), noise, seed, permutation_seed, description).graphs()
 
 
 def alignment_accuracy(ground_truth: np.ndarray, mapping) -> float:
@@ -127,14 +165,37 @@ def watts_strogatz(n: int, k: int, p: float, noise: float, seed: int = 0) -> Ins
                                       "seed": seed})
 
 
-def stochastic_block(blocks: int, block_size: int, deg_in: float, deg_out: float, noise: float,
-                     seed: int = 0) -> Instance:
+def stochastic_block_edges(blocks: int, block_size: int, deg_in: float, deg_out: float,
+                           seed: int = 0) -> np.ndarray:
+    """Edges of a stochastic block model with `blocks` equal blocks: every
+    pair inside a block is an edge with probability p_in = deg_in /
+    (block_size - 1), every pair across two blocks with p_out = deg_out /
+    (n - block_size), independently.  Per block pair the edge count is drawn
+    binomially and that many distinct pairs uniformly (the same
+    distribution), all from numpy's default_rng(seed)."""
+    rng = np.random.default_rng(seed)
     n = blocks * block_size
     p_in = deg_in / (block_size - 1)
     p_out = deg_out / (n - block_size)
-    probs = [[p_in if a == b else p_out for b in range(blocks)] for a in range(blocks)]
-    g = nx.stochastic_block_model([block_size] * blocks, probs, seed=seed, sparse=True)
-    g = nx.Graph(g)
-    return make_instance(g, NoiseLevel(0.0, noise, True), seed,
-                         description={"graph": "stochastic_block", "blocks": blocks, "block_size": block_size,
-                                      "deg_in": deg_in, "deg_out": deg_out, "noise": noise, "seed": seed})
+    iu, ju = np.triu_indices(block_size, 1)
+    parts = []
+    for a in range(blocks):
+        base = a * block_size
+        m = rng.binomial(len(iu), p_in)
+        pick = rng.choice(len(iu), size=m, replace=False)
+        parts.append(np.stack((base + iu[pick], base + ju[pick]), axis=1))
+        for b in range(a + 1, blocks):
+            m = rng.binomial(block_size * block_size, p_out)
+            pick = rng.choice(block_size * block_size, size=m, replace=False)
+            parts.append(np.stack((base + pick // block_size, b * block_size + pick % block_size), axis=1))
+    return np.concatenate(parts).astype(np.int64)
+
+
+def stochastic_block(blocks: int, block_size: int, deg_in: float, deg_out: float, noise: float,
+                     seed: int = 0) -> EdgeInstance:
+    """Stochastic block model instance (stochastic_block_edges), 1 % edge
+    noise with refill and a seeded permutation, as edge arrays."""
+    edges = stochastic_block_edges(blocks, block_size, deg_in, deg_out, seed)
+    return make_edge_instance(edges, NoiseLevel(0.0, noise, True), seed,
+                              description={"graph": "stochastic_block", "blocks": blocks, "block_size": block_size,
+                                           "deg_in": deg_in, "deg_out": deg_out, "noise": noise, "seed": seed})

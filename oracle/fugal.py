@@ -329,7 +329,9 @@ def update_quasi_permutation(P, plan, alpha) -> torch.Tensor:
 
 
 def duality_gap(gradient, step, alpha) -> float:
-    """|sum G (step / alpha)| (cugal/pred.py:140-141)."""
+    """|sum G (step / alpha)| (cugal/pred.py:140-141).  Pin: from P = J/n
+    toward a permutation plan the gap is |sum_i G[i, pi(i)] - sum G / n|
+    for every alpha; a 2 x 2 hand case."""
     return abs(torch.sum(gradient * (step / alpha)).item())
 
 

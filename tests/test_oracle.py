@@ -287,6 +287,31 @@ def test_update_keeps_doubly_stochastic():
     np.testing.assert_allclose(P.sum(1).numpy(), np.ones(n), atol=1e-9)
 
 
+@pytest.mark.parametrize("alpha", [1.0, 0.5, 2.0 / 7.0])
+def test_duality_gap_closed_forms(alpha):
+    # From P = J/n toward a permutation plan Pi, step = alpha (Pi - J/n), so
+    # the gap |sum G (step / alpha)| (cugal/pred.py:140-141) is
+    # |sum_i G[i, pi(i)] - (1/n) sum_ij G_ij|, whatever alpha is.
+    n = 9
+    g = torch.Generator().manual_seed(11)
+    G = torch.randn((n, n), dtype=torch.float64, generator=g)
+    perm = torch.randperm(n, generator=g)
+    Pi = torch.zeros((n, n), dtype=torch.float64)
+    Pi[torch.arange(n), perm] = 1.0
+    P = torch.full((n, n), 1.0 / n, dtype=torch.float64)
+    step = oracle.update_quasi_permutation(P, Pi, alpha)
+    want = abs(float(G[torch.arange(n), perm].sum()) - float(G.sum()) / n)
+    assert oracle.duality_gap(G, step, alpha) == pytest.approx(want, rel=1e-12)
+    # a hand case: G = [[1, 2], [3, 4]], P = I, plan = the swap: the gap is
+    # |(2 + 3) - (1 + 4)| = 0; with G = [[1, 5], [3, 4]] it is |8 - 5| = 3
+    P2 = torch.eye(2, dtype=torch.float64)
+    swap = torch.tensor([[0.0, 1.0], [1.0, 0.0]], dtype=torch.float64)
+    step2 = oracle.update_quasi_permutation(P2.clone(), swap, 0.25)
+    assert oracle.duality_gap(torch.tensor([[1.0, 2.0], [3.0, 4.0]], dtype=torch.float64), step2, 0.25) == 0.0
+    assert oracle.duality_gap(torch.tensor([[1.0, 5.0], [3.0, 4.0]], dtype=torch.float64), step2,
+                              0.25) == pytest.approx(3.0, rel=1e-15)
+
+
 def _brute_lap(M):
     n = M.shape[0]
     best, arg = -np.inf, None
