@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from typing import Optional
 
 import torch
@@ -169,16 +170,21 @@ def stream_of(device: torch.device) -> int:
 
 
 _workspaces: dict = {}
+_workspaces_lock = threading.Lock()
 
 
 def workspace(device: torch.device, nbytes: int, tag: str = "default") -> torch.Tensor:
-    """Grow-only device scratch buffer (uint8) from torch's allocator.  Calls
-    that share a tag on one device must not overlap."""
-    key = (str(device), tag)
-    buf = _workspaces.get(key)
-    if buf is None or buf.numel() < nbytes:
-        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
-        _workspaces[key] = buf
+    """Grow-only device scratch buffer (uint8) from torch's allocator, one
+    per (device, calling thread, current stream, tag): calls from different
+    host threads or on different streams never share one, so they may
+    overlap; calls of one thread on one stream run in stream order."""
+    dev = torch.device(device)
+    key = (str(dev), threading.get_ident(), torch.cuda.current_stream(dev).cuda_stream, tag)
+    with _workspaces_lock:
+        buf = _workspaces.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+            _workspaces[key] = buf
     return buf
 
 

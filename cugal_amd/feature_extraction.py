@@ -91,6 +91,16 @@ def extract_features_cuda(graph: nx.Graph | Adjacency, config: Config) -> torch.
     return features
 
 
+def extract_features_nx(G: nx.Graph):
+    """The reference's networkx feature path (cugal/feature_extraction.py:
+    41-64) is its CPU path, which this library does not provide: the same
+    four features come from the device kernels (extract_features_cuda,
+    node_features), equal to networkx's values to FP32 rounding."""
+    raise NotImplementedError(
+        "extract_features_nx is the reference's CPU path; use extract_features_cuda / node_features on a ROCm device"
+    )
+
+
 @dataclass
 class Features:
     """Features of source and target graph (already multiplied by mu)."""
@@ -125,10 +135,12 @@ class Features:
             raise NotImplementedError("add_distance takes the 4-feature rows of the reference kernel")
         nat.require_device_tensor(out, "out", torch.float32)
         n = out.shape[0]
-        nat.call(
-            "cg_add_distance", nat.ptr(self.source.contiguous()), nat.ptr(self.target.contiguous()), n,
-            nat.ptr(out), nat.stream_of(out.device),
-        )
+        # both contiguous copies stay referenced until the call returns (a
+        # freed first temporary could be handed to the second)
+        source, target = self.source.contiguous(), self.target.contiguous()
+        nat.require_device_tensor(source, "source features", torch.float32, 4 * n)
+        nat.require_device_tensor(target, "target features", torch.float32, 4 * n)
+        nat.call("cg_add_distance", nat.ptr(source), nat.ptr(target), n, nat.ptr(out), nat.stream_of(out.device))
         return out
 
     def distance_matrix(self) -> torch.Tensor:

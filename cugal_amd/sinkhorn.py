@@ -74,6 +74,16 @@ def scale_kernel_matrix_log(K, log_u: torch.Tensor, log_v: torch.Tensor) -> torc
     return K
 
 
+def marginal_error(K, u, v) -> float:
+    """The STANDARD form's column-marginal error (cugal/sinkhorn.py:33-40).
+    The device solve computes it inside each checking iteration and returns
+    it in SinkhornProfile.errors; there is no standalone kernel for it."""
+    raise NotImplementedError(
+        "marginal_error is computed inside the device STANDARD solve (SinkhornProfile.errors); "
+        "no standalone form is provided"
+    )
+
+
 def marginal_error_log(K, log_u: torch.Tensor, log_v: torch.Tensor) -> float:
     """sum_r |sum_c exp(K_rc + v_c + u_r) - 1| (the device kernel of
     cuda_kernels/sinkhorn_log.cu:68-98, FP64 accumulation)."""
@@ -84,8 +94,11 @@ def marginal_error_log(K, log_u: torch.Tensor, log_v: torch.Tensor) -> float:
     nat.require_device_tensor(K, "K", torch.float32)
     ws = nat.workspace(K.device, nat.query("cg_sinkhorn_log_marginal_workspace", n), "marginal")
     out = (np.zeros(1, dtype=np.float64))
+    log_u, log_v = log_u.contiguous(), log_v.contiguous()
+    nat.require_device_tensor(log_u, "log_u", torch.float32, n)
+    nat.require_device_tensor(log_v, "log_v", torch.float32, n)
     nat.call(
-        "cg_sinkhorn_log_marginal", nat.ptr(K), n, nat.ptr(log_u.contiguous()), nat.ptr(log_v.contiguous()),
+        "cg_sinkhorn_log_marginal", nat.ptr(K), n, nat.ptr(log_u), nat.ptr(log_v),
         out.ctypes.data, nat.ptr(ws), ws.numel(), nat.stream_of(K.device),
     )
     return float(out[0])

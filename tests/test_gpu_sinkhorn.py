@@ -296,3 +296,40 @@ def test_concurrent_solves_from_two_threads(device):
         assert results[k] is not None
         assert torch.equal(results[k][0], alone[k][0]) and torch.equal(results[k][1], alone[k][1])
         assert results[k][2] == alone[k][2] and 10 <= alone[k][2] < 500
+
+
+def test_concurrent_solves_through_the_python_api(device):
+    # The same through SinkhornState.solve: the library's own workspace is
+    # kept per (device, thread, stream), so two threads on their own streams
+    # solve at once and each gets what it gets alone.
+    import threading
+
+    from cugal_amd.config import Config, SinkhornMethod
+    from cugal_amd.sinkhorn import SinkhornState
+
+    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                 sinkhorn_regularization=0.4, sinkhorn_threshold=1e-3)
+
+    def solve(C, stream):
+        with torch.cuda.stream(stream):
+            state = SinkhornState(C.shape[0], cfg)
+            for _ in range(3):
+                state.solve(C, cfg)
+            stream.synchronize()
+            return state.u.cpu(), state.v.cpu()
+
+    costs = [_cost(2048, seed=33, scale=2.0).float().to(device), _cost(3000, seed=34, scale=5.0).float().to(device)]
+    alone = [solve(C, torch.cuda.Stream(device)) for C in costs]
+    results = [None, None]
+
+    def worker(k):
+        results[k] = solve(costs[k], torch.cuda.Stream(device))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for k in range(2):
+        assert results[k] is not None
+        assert torch.equal(results[k][0], alone[k][0]) and torch.equal(results[k][1], alone[k][1])

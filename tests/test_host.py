@@ -221,3 +221,58 @@ def test_experiment_results_json_round_trip(tmp_path):
     assert back.to_dict() == res.to_dict()
     assert [str(g) for g, _, _, _ in back.all_results()] == [str(g) for g in exp.graphs]
     assert res.name().startswith("NEWMAN_WATTS-CA_NETSCIENCE-")
+
+
+def test_reference_import_surface():
+    # The imports of the reference's callers (benchmark/experiment.py:8-11,
+    # test/*.py) resolve to this library's modules.
+    import importlib
+
+    import cugal
+    import cugal_amd
+    from cugal.config import Config as C2, HungarianMethod as H2, SinkhornMethod as S2
+    from cugal.pred import cugal as run, dense_gradient, sparse_gradient, update_quasi_permutation  # noqa: F401
+    from cugal.profile import Profile, TimeStamp  # noqa: F401
+    from cugal.adjacency import Adjacency  # noqa: F401
+    from cugal.feature_extraction import Features, extract_features_cuda, extract_features_nx  # noqa: F401
+    from cugal.greedy_lap import greedy_lap  # noqa: F401
+    from cugal.sinkhorn import SinkhornState, marginal_error  # noqa: F401
+
+    assert run is cugal_amd.pred.cugal and cugal.cugal is run
+    assert (C2, H2, S2) == (cugal_amd.Config, cugal_amd.HungarianMethod, cugal_amd.SinkhornMethod)
+    for name in ("pred", "config", "profile", "sinkhorn", "adjacency", "feature_extraction", "greedy_lap"):
+        assert importlib.import_module(f"cugal.{name}") is importlib.import_module(f"cugal_amd.{name}")
+    with pytest.raises(NotImplementedError):
+        extract_features_nx(None)
+
+
+def test_cuda_kernels_module_has_the_reference_exports():
+    # cuda_kernels/kernels.cpp:32-70: ten functions, each on a library
+    # entry point the binding declares.
+    import inspect
+
+    import cuda_kernels
+
+    assert len(cuda_kernels.EXPORTS) == 10
+    arity = {"adjacency_matmul": 5, "create_adjacency": 3, "graph_features": 4, "average_neighbor_features": 4,
+             "sinkhorn_log_step": 4, "sinkhorn_log_marginal": 3, "add_distance": 3, "regularize": 3,
+             "update_quasi_permutation_log": 6, "dense_hungarian": 2}
+    for name in cuda_kernels.EXPORTS:
+        assert len(inspect.signature(getattr(cuda_kernels, name)).parameters) == arity[name], name
+    src = inspect.getsource(importlib_module("cugal_amd.cuda_kernels"))
+    for entry in ("cg_adjacency_matmul", "cg_csr_build", "cg_graph_features", "cg_average_neighbor_features",
+                  "cg_sinkhorn_log_step", "cg_sinkhorn_log_marginal", "cg_add_distance", "cg_regularize",
+                  "cg_update_quasi_permutation_log", "cg_lap_hungarian"):
+        assert f'"{entry}"' in src and entry in nat_symbols()
+
+
+def importlib_module(name):
+    import importlib
+
+    return importlib.import_module(name)
+
+
+def nat_symbols():
+    from cugal_amd import _native
+
+    return _native.exported_symbols()
