@@ -1,7 +1,9 @@
 """Configuration of the alignment, field-for-field the surface of
 cugal/config.py:11-141 (same enum names and values, same fields, defaults,
-order and dict round trip), plus two extension fields at the end whose
-defaults do not change results.
+order and dict round trip), plus four extension fields at the end: the
+Sinkhorn sweep structure, the gradient's arithmetic, the node-feature set
+and the labels the loop runs on.  Their defaults give the reference's
+results up to the rounding of FP32 sums.
 """
 
 from __future__ import annotations
@@ -132,6 +134,14 @@ class Config:
     cugal/feature_extraction.py:41-64) or "fugal7" (the 7 of the original
     FUGAL, FUGAL/pred.py:24-110 with simple=False: the same 4 plus ego-net
     edges, edges leaving the ego net and nodes at distance 2)."""
+
+    node_order: str = "auto"
+    """Labels the Frank-Wolfe loop runs on: "natural" (the graphs' own),
+    "rcm" (each graph relabelled by reverse Cuthill-McKee, so the CSR
+    gathers of the gradient read nearby rows) or "auto" (RCM for a graph
+    where it keeps clearly more edges local, from n = 8192, on the CSR
+    gradient form).  The quasi-permutation is returned in the graphs' own
+    labels; a relabelling changes only the order of floating-point sums."""
 
     def convert_tensor(self, input: torch.Tensor) -> torch.Tensor:
         """Convert tensor to correct type and dtype."""

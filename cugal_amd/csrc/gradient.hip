@@ -292,6 +292,60 @@ __global__ __launch_bounds__(kSpThreads) void copy_rows_padded(const float* __re
     for (int j = threadIdx.x * 4; j < n; j += kSpThreads * 4) st_stream4(dst + j, ld_stream4(src + j));
 }
 
+// dst[i][j] = src[row_map[i]][col_map[j]] (a null map is the identity): one
+// block per output row; the source row is read once through L2 by the
+// gathered column loads, the output row is written with 16-byte stores.
+template <bool VEC>
+__global__ __launch_bounds__(kSpThreads) void permute_rows_cols(const float* __restrict__ src, int64_t lds, int n,
+                                                               const int32_t* __restrict__ row_map,
+                                                               const int32_t* __restrict__ col_map,
+                                                               float* __restrict__ dst, int64_t ldd) {
+    const int i = blockIdx.x;
+    const int si = row_map ? row_map[i] : i;
+    const float* __restrict__ srow = src + static_cast<int64_t>(si) * lds;
+    float* __restrict__ drow = dst + static_cast<int64_t>(i) * ldd;
+    if (VEC) {
+        for (int j = threadIdx.x * 4; j < n; j += kSpThreads * 4) {
+            float4 o;
+            if (col_map) {
+                const int4 c = *reinterpret_cast<const int4*>(col_map + j);
+                o = make_float4(srow[c.x], srow[c.y], srow[c.z], srow[c.w]);
+            } else {
+                o = *reinterpret_cast<const float4*>(srow + j);
+            }
+            st_stream4(drow + j, o);
+        }
+    } else {
+        for (int j = threadIdx.x; j < n; j += kSpThreads) drow[j] = srow[col_map ? col_map[j] : j];
+    }
+}
+
+// The same copy with the source row staged in LDS (n floats, n <= the LDS a
+// block may hold): the row is read with coalesced 16-byte loads and the
+// column gather runs on LDS instead of issuing a cache-line request per
+// element.  Blocks stride over the rows.
+constexpr int kPermThreads = 1024;
+
+__global__ __launch_bounds__(kPermThreads) void permute_rows_cols_lds(const float* __restrict__ src, int64_t lds,
+                                                                     int n, const int32_t* __restrict__ row_map,
+                                                                     const int32_t* __restrict__ col_map,
+                                                                     float* __restrict__ dst, int64_t ldd) {
+    extern __shared__ float srow_lds[];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int si = row_map ? row_map[i] : i;
+        const float* __restrict__ srow = src + static_cast<int64_t>(si) * lds;
+        for (int j = threadIdx.x * 4; j < n; j += kPermThreads * 4)
+            *reinterpret_cast<float4*>(srow_lds + j) = ld_stream4(srow + j);
+        __syncthreads();
+        float* __restrict__ drow = dst + static_cast<int64_t>(i) * ldd;
+        for (int j = threadIdx.x * 4; j < n; j += kPermThreads * 4) {
+            const int4 c = *reinterpret_cast<const int4*>(col_map + j);
+            st_stream4(drow + j, make_float4(srow_lds[c.x], srow_lds[c.y], srow_lds[c.z], srow_lds[c.w]));
+        }
+        __syncthreads();
+    }
+}
+
 // ---- elementwise reference-compatible helpers ----------------------------
 __global__ void add_distance_kernel(const float* __restrict__ s, const float* __restrict__ t, int n,
                                     float* __restrict__ out) {
@@ -310,7 +364,7 @@ __global__ void regularize_kernel(float* __restrict__ g, const float* __restrict
     g[idx] += lam - lam * 2.0f * P[idx];
 }
 
-[[maybe_unused]] int max_lds_bytes() {
+int max_lds_bytes() {
     static int cached = -1;
     if (cached < 0) {
         int dev = 0, v = 0;
@@ -396,6 +450,34 @@ int cg_add_distance(const float* source, const float* target, int64_t n, float* 
     hipLaunchKernelGGL(add_distance_kernel, dim3(static_cast<unsigned>(div_up(total, 256))), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), source, target, static_cast<int>(n), out);
     return check_launch("add_distance_kernel");
+}
+
+int cg_permute_copy(const float* src, int64_t ld_src, int64_t n, const int32_t* row_map, const int32_t* col_map,
+                    float* dst, int64_t ld_dst, cg_stream_t stream) {
+    if (!src || !dst || bad_n(n) || ld_src < n || ld_dst < n) return fail(CG_EINVAL, "cg_permute_copy: bad arguments");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const bool vec = n % 4 == 0 && ld_dst % 4 == 0 && aligned16(dst) && (!col_map || aligned16(col_map)) &&
+                     (col_map || (ld_src % 4 == 0 && aligned16(src)));
+    const size_t row_bytes = sizeof(float) * static_cast<size_t>(n);
+    if (vec && col_map && ld_src % 4 == 0 && aligned16(src) && row_bytes + 1024 <= static_cast<size_t>(max_lds_bytes())) {
+        CG_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&permute_rows_cols_lds),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(row_bytes)));
+        int dev = 0, cus = 0;
+        CG_CHECK_HIP(hipGetDevice(&dev));
+        CG_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t per_cu = std::max<int64_t>(1, static_cast<int64_t>(max_lds_bytes()) / (row_bytes + 1024));
+        const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(n, std::max(1, cus) * std::min<int64_t>(per_cu, 2)));
+        hipLaunchKernelGGL(permute_rows_cols_lds, dim3(blocks), dim3(kPermThreads), row_bytes, s, src, ld_src,
+                           static_cast<int>(n), row_map, col_map, dst, ld_dst);
+        return check_launch("permute_rows_cols_lds");
+    }
+    if (vec)
+        hipLaunchKernelGGL(permute_rows_cols<true>, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, src,
+                           ld_src, static_cast<int>(n), row_map, col_map, dst, ld_dst);
+    else
+        hipLaunchKernelGGL(permute_rows_cols<false>, dim3(static_cast<unsigned>(n)), dim3(kSpThreads), 0, s, src,
+                           ld_src, static_cast<int>(n), row_map, col_map, dst, ld_dst);
+    return check_launch("permute_rows_cols");
 }
 
 int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_stream_t stream) {

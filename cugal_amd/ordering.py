@@ -1,0 +1,122 @@
+"""Node orders that make the gradient's CSR gathers local.
+
+The gradient -2 A P B gathers, for each output row, the rows of P (then of
+the intermediate) that the row's neighbour list names.  A thread block
+gathers a 64-row tile of one column strip and the blocks of one XCD sweep
+consecutive tiles, so the strip's lines are reused from that XCD's L2 when
+neighbours have nearby labels.  A graph whose labels are scattered (a
+relabelled copy, as every alignment target is) fetches each line many times.
+
+Relabelling a graph changes no result of the alignment beyond the order of
+floating-point sums: with permutations T (source) and S (target),
+T A T^T (T P S^T) S B S^T = T (A P B) S^T, the distance and regulariser terms
+are elementwise, and Sinkhorn and the Frank-Wolfe update commute with
+relabelling rows and columns.  The loop therefore runs on relabelled graphs
+and its quasi-permutation is copied back to the callers' labels
+(cg_permute_copy) before the LAP, which sees the original labels.
+
+The order is reverse Cuthill-McKee (breadth-first levels, neighbours of
+earlier nodes first), computed on the host from the CSR; "auto" keeps it for
+a graph only where it puts clearly more edges within a window of labels than
+the graph's own labels do.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse
+import scipy.sparse.csgraph
+import torch
+
+from . import _native as nat
+from .adjacency import Adjacency, edges_to_csr
+
+ORDERS = ("auto", "natural", "rcm")
+
+# "auto" relabels only from this size: below it a column strip of the
+# gathered matrix is a fraction of an XCD's L2 in any order.
+AUTO_MIN_NODES = 8192
+# ... and only when the order puts this many more of the edges within the
+# window (the fraction of edges whose endpoints' labels differ by at most
+# min(n / 4, 8192) rows: 1 MiB of a 128-byte-per-row strip).
+AUTO_MIN_GAIN = 0.05
+
+
+def _host_csr(adj: Adjacency):
+    rp = adj.row_pointers.cpu().numpy().astype(np.int64)
+    col = adj.col_indices.cpu().numpy().astype(np.int64)
+    return rp, col
+
+
+def window_locality(rp: np.ndarray, col: np.ndarray, position: np.ndarray | None = None) -> float:
+    """Fraction of CSR entries (r, c) with |pos(r) - pos(c)| <= min(n/4,
+    8192); `position` maps each node to its label (identity when None)."""
+    n = len(rp) - 1
+    if len(col) == 0:
+        return 1.0
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    if position is not None:
+        rows, cols = position[rows], position[col]
+    else:
+        cols = col
+    window = min(max(n // 4, 1), 8192)
+    return float(np.mean(np.abs(rows - cols) <= window))
+
+
+def rcm_position(rp: np.ndarray, col: np.ndarray) -> np.ndarray:
+    """New label of each node under reverse Cuthill-McKee."""
+    n = len(rp) - 1
+    M = scipy.sparse.csr_matrix((np.ones(len(col), dtype=np.int8), col, rp), shape=(n, n))
+    order = scipy.sparse.csgraph.reverse_cuthill_mckee(M, symmetric_mode=True)
+    position = np.empty(n, dtype=np.int64)
+    position[order] = np.arange(n, dtype=np.int64)
+    return position
+
+
+def choose_position(adj: Adjacency, mode: str) -> np.ndarray | None:
+    """The relabelling of `adj` for the gathers (new label of each node), or
+    None to keep its labels."""
+    if mode not in ORDERS:
+        raise ValueError(f"node_order must be one of {ORDERS}, got {mode!r}")
+    n = adj.number_of_nodes()
+    if mode == "natural" or n < 2 or (mode == "auto" and n < AUTO_MIN_NODES):
+        return None
+    rp, col = _host_csr(adj)
+    position = rcm_position(rp, col)
+    if mode == "auto" and window_locality(rp, col, position) < window_locality(rp, col) + AUTO_MIN_GAIN:
+        return None
+    return position
+
+
+def relabel(adj: Adjacency, position: torch.Tensor) -> Adjacency:
+    """The CSR of the graph with node v renamed position[v] (position an
+    int64 device tensor, a permutation of 0..n-1); rows and the sorted
+    column order are rebuilt on the device, repeated entries kept."""
+    n = adj.number_of_nodes()
+    dev = adj.row_pointers.device
+    counts = (adj.row_pointers[1:] - adj.row_pointers[:-1]).to(torch.int64)
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), counts)
+    cols = adj.col_indices.to(torch.int64)
+    # a pair (x, y) is the entry [y][x]
+    edges = torch.stack((position[cols], position[rows]), dim=1).to(torch.int32)
+    col, rp = edges_to_csr(edges, n, symmetrize=False)
+    return Adjacency(col, rp, loops_twice=adj.loops_twice)
+
+
+def permute_rows(features: torch.Tensor, position: torch.Tensor) -> torch.Tensor:
+    """Row position[v] of the result is row v of `features`."""
+    out = torch.empty_like(features)
+    out[position] = features
+    return out
+
+
+def copy_back(P: torch.Tensor, row_position: torch.Tensor | None, col_position: torch.Tensor | None) -> torch.Tensor:
+    """Contiguous Q with Q[i][j] = P[row_position[i]][col_position[j]]: the
+    quasi-permutation of relabelled graphs in the original labels."""
+    n = P.shape[0]
+    out = torch.empty((n, n), dtype=P.dtype, device=P.device)
+    rmap = None if row_position is None else row_position.to(torch.int32).contiguous()
+    cmap = None if col_position is None else col_position.to(torch.int32).contiguous()
+    nat.call("cg_permute_copy", nat.ptr(P), nat.row_stride(P, "P"), n, nat.ptr(rmap), nat.ptr(cmap), nat.ptr(out), n,
+             nat.stream_of(P.device))
+    return out

@@ -21,6 +21,7 @@ import scipy.sparse.csgraph
 import torch
 
 from . import _native as nat
+from . import ordering
 from .adjacency import Adjacency
 from .config import Config, HungarianMethod, SinkhornMethod
 from .feature_extraction import Features
@@ -44,6 +45,8 @@ def _check_config(config: Config) -> None:
         )
     if config.gradient_precision not in ("fp32", "bf16x2", "auto"):
         raise NotImplementedError(f"gradient_precision {config.gradient_precision!r} is not available")
+    if config.node_order not in ordering.ORDERS:
+        raise ValueError(f"node_order must be one of {ordering.ORDERS}, got {config.node_order!r}")
 
 
 # Density above which the matrix-core gradient beats the CSR gathers on one
@@ -223,6 +226,24 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
     if B.number_of_nodes() != n:
         raise ValueError(f"graphs differ in size ({n} vs {B.number_of_nodes()})")
     dev = A.row_pointers.device
+    precision = gradient_precision_for(A, B, config.gradient_precision)
+    # The CSR gathers run on graphs relabelled for locality where that
+    # pays; P is copied back to the original labels at the end.
+    row_position = col_position = None
+    if precision == "fp32" and (features is None or isinstance(features, Features)):
+        pa = ordering.choose_position(A, config.node_order)
+        pb = ordering.choose_position(B, config.node_order)
+        if pa is not None:
+            row_position = torch.from_numpy(pa).to(dev)
+            A = ordering.relabel(A, row_position)
+        if pb is not None:
+            col_position = torch.from_numpy(pb).to(dev)
+            B = ordering.relabel(B, col_position)
+        if features is not None and (pa is not None or pb is not None):
+            features = Features(
+                features.source if pa is None else ordering.permute_rows(features.source, row_position),
+                features.target if pb is None else ordering.permute_rows(features.target, col_position),
+            )
     sinkhorn_state = SinkhornState(n, config)
     # P lives in a row-padded buffer during the loop: the gradient gathers
     # its rows in place (a 2^k-float row stride would alias L2 sets and cost
@@ -236,7 +257,7 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
     for lam in range(config.iter_count):
         for it in range(1, config.frank_wolfe_iter_count + 1):
             start_time = TimeStamp(config.device)
-            _gradient_into(gradient, A, B, P, features, lam, factor, config.gradient_precision)
+            _gradient_into(gradient, A, B, P, features, lam, factor, precision)
             profile.log_time(start_time, Phase.GRADIENT)
 
             start_time = TimeStamp(config.device)
@@ -254,6 +275,8 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
             profile.frank_wolfe_iterations += 1
             if config.frank_wolfe_threshold is not None and abs(float(gap.item())) < config.frank_wolfe_threshold:
                 break
+    if row_position is not None or col_position is not None:
+        return ordering.copy_back(P, row_position, col_position)
     return P.contiguous()
 
 

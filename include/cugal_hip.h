@@ -147,6 +147,16 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
                 int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
                 size_t workspace_bytes, cg_stream_t stream);
 
+/* dst[i][j] = src[row_map[i]][col_map[j]] for n x n FP32 matrices with row
+ * strides ld_src, ld_dst (a NULL map is the identity; maps hold values in
+ * 0..n-1, which the caller guarantees).  No reference counterpart: the
+ * Frank-Wolfe loop may run on relabelled graphs (a locality order of their
+ * CSR rows, cugal/pred.py:89-148 is unchanged by a relabelling up to the
+ * order of floating-point sums), and this copies its quasi-permutation back
+ * to the callers' labels. */
+int cg_permute_copy(const float* src, int64_t ld_src, int64_t n, const int32_t* row_map, const int32_t* col_map,
+                    float* dst, int64_t ld_dst, cg_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * Sinkhorn (log domain, FP32).
  * ---------------------------------------------------------------------- */

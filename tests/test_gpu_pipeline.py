@@ -74,12 +74,15 @@ def test_isomorphic_pair_recovered(device):
     assert alignment_accuracy(inst.source_mapping, answer_columns(mapping)) == 1.0
 
 
-def test_fixed_iteration_quasi_permutation_matches_oracle(device):
+@pytest.mark.parametrize("order", ["natural", "rcm"])
+def test_fixed_iteration_quasi_permutation_matches_oracle(device, order):
     # With a zero threshold every solve runs its full iteration count, so the
-    # FP32 trajectory can be compared entry by entry with the FP64 one.
+    # FP32 trajectory can be compared entry by entry with the FP64 one; with
+    # "rcm" the loop runs on relabelled graphs and P comes back in the
+    # graphs' own labels.
     inst = erdos_renyi(200, 0.05, 0.05, seed=4)
     gpu, ref = _configs(True, 1.0, 1.0, 2, 3, lap="SCIPY", thr=0.0)
-    gpu = Config(**{**gpu.__dict__, "sinkhorn_iterations": 30})
+    gpu = Config(**{**gpu.__dict__, "sinkhorn_iterations": 30, "node_order": order})
     ref.sinkhorn_iterations = 30
     P, mapping = cugal(copy.deepcopy(inst.source), copy.deepcopy(inst.target), gpu, Profile())
     run = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), ref)

@@ -70,7 +70,7 @@ def test_config_fields_match_reference():
     for name, value in REFERENCE_FIELDS:
         assert getattr(default, name) == value, name
     # extension fields do not change results by default
-    assert names[len(REFERENCE_FIELDS):] == ["sinkhorn_fused", "gradient_precision", "node_features"]
+    assert names[len(REFERENCE_FIELDS):] == ["sinkhorn_fused", "gradient_precision", "node_features", "node_order"]
 
 
 def test_config_enums_match_reference():
@@ -86,7 +86,8 @@ def test_config_dict_round_trip():
     assert d["dtype"] == "float32" and d["sinkhorn_method"] == "LOG" and d["hungarian_method"] == "SCIPY"
     assert Config.from_dict(d) == c
     # a dict written by the reference (no extension keys) loads too
-    ref = {k: v for k, v in d.items() if k not in ("sinkhorn_fused", "gradient_precision")}
+    ref = {k: v for k, v in d.items() if k not in ("sinkhorn_fused", "gradient_precision", "node_features",
+                                                    "node_order")}
     assert Config.from_dict(ref) == c
 
 
@@ -276,3 +277,36 @@ def nat_symbols():
     from cugal_amd import _native
 
     return _native.exported_symbols()
+
+
+def test_locality_order_choice():
+    # A ring lattice under a random relabelling has almost no edges within
+    # the label window; reverse Cuthill-McKee restores most of them, and
+    # "auto" keeps the ring's own labels where they are already local.
+    import networkx as nx
+    import numpy as np
+
+    from cugal_amd import ordering
+    from cugal_amd.adjacency import Adjacency
+
+    n = 16384
+    ring = nx.watts_strogatz_graph(n, 10, 0.0, seed=0)
+    shuffled = nx.relabel_nodes(ring, dict(zip(range(n), np.random.default_rng(1).permutation(n).tolist())))
+
+    def cpu_adj(g):
+        M = nx.to_scipy_sparse_array(g, nodelist=range(g.number_of_nodes()), format="csr")
+        M.sort_indices()
+        return Adjacency(torch.from_numpy(M.indices.astype(np.int32)), torch.from_numpy(M.indptr.astype(np.int32)))
+
+    a, b = cpu_adj(ring), cpu_adj(shuffled)
+    rp, col = ordering._host_csr(b)
+    pos = ordering.rcm_position(rp, col)
+    assert np.array_equal(np.sort(pos), np.arange(n))
+    assert ordering.window_locality(rp, col) < 0.6 < 0.99 < ordering.window_locality(rp, col, pos)
+    assert ordering.choose_position(a, "auto") is None
+    assert ordering.choose_position(b, "auto") is not None
+    assert ordering.choose_position(b, "natural") is None
+    small = cpu_adj(nx.relabel_nodes(nx.cycle_graph(100), dict(zip(range(100), range(99, -1, -1)))))
+    assert ordering.choose_position(small, "auto") is None and ordering.choose_position(small, "rcm") is not None
+    with pytest.raises(ValueError):
+        ordering.choose_position(a, "bfs")
