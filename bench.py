@@ -14,28 +14,34 @@ Reported: `value` = seconds per alignment on the "mixed" path -- the fused
 Sinkhorn (the cost read once per iteration; FP32 potentials, FP64 column
 sums, marginal errors and Frank-Wolfe gap) with the gradient in the form
 the adjacency density calls for (`gradient_precision="auto"`: the CSR
-gathers here, the matrix cores for dense graphs) -- beside the plain FP32
-path (two-sweep Sinkhorn, CSR gradient) on the same input, node accuracies, the Sinkhorn sweep roofline (bytes =
-4 n^2 per sweep over the HIP-event time of the sweep launches of every 10th
-iteration, recorded on the launch stream inside the timed run) and the FP64
-CPU oracle timed on a bounded sample of the same Sinkhorn work.  The
-precision form (gradient on the matrix cores: 0/1 BF16 adjacency times BF16
-hi + lo P) runs once on the same input (`precision_form`: time, accuracy,
-matrix-core rate, and the gradient's error against FP64 on sampled rows),
-and so does the differentiation form (`differentiation_form`: Erdos-Renyi
-n=4096, 100 unrolled log-Sinkhorn iterations with checkpoints every 10;
-forward and hand-written backward times and the gradients' error against
-torch autograd in FP64).  `reference_case` is the reference's own
-CPU-runnable case (Erdos-Renyi n=1000 with its default Config), cugal() on
-the device and the FP64 oracle on the host cores, each timed end to end.
+gathers here, the matrix cores for dense graphs) on graphs relabelled for
+gather locality where that pays (`node_order="auto"`) -- beside the plain
+FP32 path (two-sweep Sinkhorn, CSR gradient) on the same input, node
+accuracies, the Sinkhorn sweep roofline (bytes = 4 n^2 per sweep over the
+HIP-event time of the sweep launches of every 10th iteration, recorded on
+the launch stream inside the timed run) and the FP64 CPU oracle timed on a
+bounded sample of the same Sinkhorn work.  `precision_form` gives the
+matrix-core gradient's time and rate, the alignment time it implies (and,
+when the budget allows, the measured whole alignment), and every stage's
+error against FP64 at this size from the alignment's own P: gradient in
+both forms, Sinkhorn plan and marginal error, update and duality gap.
+Further legs: `differentiation_form` (Erdos-Renyi n=4096, 100 unrolled
+log-Sinkhorn iterations with checkpoints every 10; forward and hand-written
+backward times and the gradients' error against torch autograd in FP64),
+`reference_case` (the reference's own CPU-runnable case, Erdos-Renyi
+n=1000 with its default Config, cugal() on the device and the FP64 oracle
+on the host cores), `ba8192` and `sbm65536` (below).  The run keeps to a
+wall-clock budget (`--budget-s`, counted from the interpreter's start):
+after the timed steps each leg runs only if its estimate fits and is
+otherwise recorded as skipped, so the JSON line always prints.
 
-Other workloads (`--workload`): `ba8192` (Barabasi-Albert n=8192, m=5, 2%
-edge noise; the LAP as greedy and as the exact device Hungarian, checked
-against scipy on the same P; the FP64 oracle timed per stage and
-extrapolated) and `sbm65536` (stochastic block model, 64
-blocks of 1024, intra/inter degrees 48/16, 1% noise, 15 lambda stages of one
-Frank-Wolfe step, greedy LAP; P is 17 GB) and `er4096grad` (the
-differentiation form alone).
+Other workloads (`--workload`, also run as legs of the default run):
+`ba8192` (Barabasi-Albert n=8192, m=5, 2% edge noise; the LAP as greedy and
+as the exact device Hungarian, checked against scipy on the same P; alone,
+also the FP64 oracle timed per stage and extrapolated) and `sbm65536`
+(stochastic block model, 64 blocks of 1024, intra/inter degrees 48/16, 1%
+noise, 15 lambda stages of one Frank-Wolfe step, greedy LAP; P is 17 GB);
+`er4096grad` (the differentiation form alone).
 """
 
 from __future__ import annotations
@@ -47,6 +53,7 @@ import os
 import sys
 import time
 
+T_START = time.time()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -68,6 +75,7 @@ def make_config(fused: bool, args, precision: str = "auto", lap: str = "DOUBLE_G
         sinkhorn_threshold=args.threshold, sinkhorn_eval_freq=10, mu=args.mu, iter_count=args.iter_count,
         frank_wolfe_iter_count=args.fw_iter_count, use_sparse_adjacency=True,
         hungarian_method=HungarianMethod[lap], sinkhorn_fused=fused, gradient_precision=precision,
+        node_order=getattr(args, "node_order", "auto"),
     )
 
 
@@ -420,6 +428,145 @@ def probe(args) -> int:
     return 0
 
 
+class Budget:
+    """Wall clock of the whole run, counted from the interpreter's start
+    (torch's first import included), against `seconds`: the optional legs
+    after the timed steps run only while their estimate fits, so the JSON
+    line always prints in time."""
+
+    def __init__(self, seconds: float):
+        self.seconds = seconds
+
+    def left(self) -> float:
+        return self.seconds - (time.time() - T_START)
+
+    def allows(self, name: str, needs: float) -> bool:
+        if self.left() >= needs:
+            return True
+        log(f"skipping {name}: {self.left():.0f} s of the wall-clock budget left, it needs ~{needs:.0f} s")
+        return False
+
+    def skipped(self, needs: float) -> dict:
+        return {"skipped": f"wall-clock budget: {max(0.0, self.left()):.0f} s left of {self.seconds:.0f}, "
+                           f"leg needs ~{needs:.0f} s"}
+
+
+def _fp64_checks():
+    """The FP64 device checkers shared with the tests (tests/fp64_checks.py)."""
+    path = os.path.join(ROOT, "tests")
+    if path not in sys.path:
+        sys.path.insert(0, path)
+    import fp64_checks
+
+    return fp64_checks
+
+
+def stage_errors(inputs, P, args, rows: int = 16, seed: int = 0):
+    """Error of each stage against FP64 at the workload's size, from the
+    alignment's own quasi-permutation P: the gradient in both forms (with
+    distance and regulariser terms, lambda 7) on sampled rows; ten
+    iterations of the fused Sinkhorn solve from zero duals, the plan on
+    sampled rows and the column-marginal error against FP64 iterations on
+    the device; the Frank-Wolfe update (alpha 2/3) and its duality gap
+    against FP64 over the whole matrix."""
+    import numpy as np
+    import torch
+
+    from cugal_amd import _native as nat
+    from cugal_amd.feature_extraction import Features, node_features
+    from cugal_amd.pred import _gradient_into, update_quasi_permutation_log
+    from cugal_amd.sinkhorn import LogKernel, solve_log_native
+
+    f64 = _fp64_checks()
+    A, B = inputs
+    n = P.shape[0]
+    dev = P.device
+    feats = Features(node_features(A, args.mu), node_features(B, args.mu))
+    ld = int(nat.query("cg_gradient_padded_ld", n))
+    Pp = torch.empty((n, ld), device=dev)[:, :n]
+    Pp.copy_(P)
+    rng = np.random.default_rng(seed)
+    sample = sorted({0, n - 1, *[int(r) for r in rng.choice(n, size=rows, replace=False)]})
+    idx = torch.as_tensor(sample, device=dev)
+    lam = 7
+    ref, prod, other = f64.gradient_rows(A, B, Pp, feats, lam, 1.0, sample)
+    G = torch.empty((n, n), device=dev)
+    out = {"rows": len(sample)}
+    for form in ("bf16x2", "fp32"):
+        _gradient_into(G, A, B, Pp, feats, lam, 1.0, precision=form)
+        got = G[idx].double().cpu().numpy()
+        err = np.abs(got - ref)
+        out[f"gradient_{form}"] = {
+            "max_abs_err": float(err.max()),
+            "max_rel_err_of_row_max": float(np.max(err.max(1) / np.abs(ref).max(1))),
+            "max_err_over_rounding_bound": float(np.max(err / (1e-5 * prod + 1e-6 * other + 1e-30))),
+        }
+    reg = args.reg
+    u = torch.zeros(n, device=dev)
+    v = torch.zeros(n, device=dev)
+    iters = 10
+    errors, _, _ = solve_log_native(G, reg, u, v, iters + 1, 0.0, 10, None, fused=True)
+    u64, v64 = f64.log_sinkhorn(G, reg, torch.zeros(n, device=dev), torch.zeros(n, device=dev), iters + 1)
+    got = f64.plan_rows(G, reg, u, v, sample)
+    want = f64.plan_rows(G, reg, u64, v64, sample)
+    big = want > 1e-200
+    _, col = f64.marginals(G, reg, u, v)
+    err64 = float((col - 1).abs().sum())
+    out["sinkhorn"] = {
+        "iterations": iters + 1, "regularization": reg,
+        "plan_max_rel_err": float(np.max(np.abs(got - want)[big] / want[big])),
+        "plan_exponent_floor": f64.exponent_floor(float(G.abs().max()), reg),
+        "marginal_error_device": errors[-1], "marginal_error_fp64_of_same_duals": err64,
+    }
+    alpha = 2.0 / 3.0
+    P_old = Pp.clone()
+    gap = torch.empty(1, dtype=torch.float64, device=dev)
+    update_quasi_permutation_log(Pp, LogKernel(G, reg), u, v, alpha, gap=gap)
+    plan = f64.plan_rows(G, reg, u, v, sample)
+    old = P_old[idx].double().cpu().numpy()
+    want_p = old + alpha * (plan - old)
+    s_plan, a_plan = f64.plan_inner(G, reg, u, v)
+    s_old, a_old = f64.inner(G, P_old)
+    gap64 = s_plan - s_old
+    err_p = np.abs(Pp[idx].double().cpu().numpy() - want_p)
+    out["update"] = {
+        "alpha": alpha,
+        "P_max_err_of_row_max": float(np.max(err_p.max(1) / np.abs(want_p).max(1))),
+        "gap_device": float(gap), "gap_fp64": gap64,
+        # the gap is a difference of two sums of size sum|G plan|; each plan
+        # entry carries the exponent floor, so its error scales with that sum
+        "gap_abs_err_over_sum_abs_G_plan": abs(float(gap) - gap64) / a_plan,
+        "sum_abs_G_plan": a_plan, "sum_abs_G_P": a_old,
+    }
+    out["product_only"] = gradient_error_sample(A, B, P_old, rows=rows, seed=seed)
+    out["reference"] = "FP64 on the device from the same FP32 inputs (gradient rows by CSR products, " \
+                       "log-Sinkhorn by chunked logsumexp, update and gap by chunked sums)"
+    return out
+
+
+def mfma_gradient_seconds(inputs, n: int, repeats: int = 3) -> float:
+    """Seconds per matrix-core gradient (HIP events on the launch stream)."""
+    import torch
+
+    from cugal_amd import _native as nat
+    from cugal_amd.pred import _gradient_into
+
+    A, B = inputs
+    dev = torch.device("cuda:0")
+    ld = int(nat.query("cg_gradient_padded_ld", n))
+    P = torch.empty((n, ld), device=dev)[:, :n]
+    P.fill_(1.0 / n)
+    G = torch.empty((n, n), device=dev)
+    _gradient_into(G, A, B, P, None, 3, 1.0, precision="bf16x2")
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        _gradient_into(G, A, B, P, None, 3, 1.0, precision="bf16x2")
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / repeats / 1e3
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", action="store_true", help="launch each hot kernel a few times (for profilers)")
@@ -437,12 +584,16 @@ def main() -> int:
     ap.add_argument("--sinkhorn-iterations", type=int, default=500)
     ap.add_argument("--iter-count", type=int, default=15)
     ap.add_argument("--fw-iter-count", type=int, default=10)
+    ap.add_argument("--node-order", choices=("auto", "natural", "rcm"), default="auto")
     ap.add_argument("--workload", choices=("ws32768", "ba8192", "sbm65536", "er4096grad"), default="ws32768")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form (backward) run")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain two-sweep comparison run")
     ap.add_argument("--no-precision-form", action="store_true", help="skip the matrix-core gradient run")
+    ap.add_argument("--no-extra-workloads", action="store_true", help="skip the BA n=8192 and SBM n=65536 legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--budget-s", type=float, default=560.0,
+                    help="wall-clock seconds the whole run may take; optional legs are skipped past it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -458,15 +609,20 @@ def main() -> int:
     if args.probe:
         return probe(args)
     if args.workload == "ba8192":
-        return run_ba8192(args)
+        extra = ba8192_leg(args, cpu=not args.no_cpu)
+        print(_contract_line(extra.pop("seconds"), args, extra.pop("workload"), 8192, extra), flush=True)
+        return 0
     if args.workload == "sbm65536":
-        return run_sbm65536(args)
+        extra = sbm65536_leg(args, precision_form=not args.no_precision_form)
+        print(_contract_line(extra.pop("seconds"), args, extra.pop("workload"), extra.pop("n"), extra), flush=True)
+        return 0
     if args.workload == "er4096grad":
         diff = differentiation_form(args.seed)
         log(f"differentiation form: forward {diff['forward_ms']:.1f} ms, backward {diff['backward_ms']:.1f} ms")
         print(_contract_line(diff["backward_ms"] * 1e-3, args, diff["workload"] + " (value = backward seconds)",
                              diff["n"], {"differentiation_form": diff}), flush=True)
         return 0
+    budget = Budget(args.budget_s)
     log(f"device {torch.cuda.get_device_name(0)}; generating WS(n={args.n}, k={args.k}, p={args.p})")
     t = time.perf_counter()
     inst = watts_strogatz(args.n, args.k, args.p, args.noise, seed=args.seed)
@@ -496,42 +652,103 @@ def main() -> int:
         iters_total = sum(p.iteration_count for p in profile.sinkhorn_profiles)
         log(f"step {s}: fused {e:.2f}s acc {a:.4f} sinkhorn iterations {iters_total}")
     value = sum(times) / len(times)
+    step_s = max(times)
     api_extra = api_extra_seconds(inst, P, "cuda:0")
-    P = None
+
+    from cugal_amd.pred import gradient_precision_for
+    from cugal_amd.profile import Phase
+
+    form = gradient_precision_for(inputs[0], inputs[1], "auto")
+    grad_s = profile.phase_times.get(Phase.GRADIENT, 0.0)
+    fw_steps = profile.frank_wolfe_iterations
 
     plain = None
     if not args.no_plain:
-        e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args, precision="fp32"), True, inputs)
-        plain = {"seconds": e, "accuracy": a,
-                 "sinkhorn_iterations": sum(p.iteration_count for p in pprof.sinkhorn_profiles),
-                 "sweep_ms": pms, "sweep_launches": pcnt}
-        log(f"plain two-sweep: {e:.2f}s acc {a:.4f}")
+        if budget.allows("plain", 2.1 * step_s + 10):
+            e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args, precision="fp32"), True, inputs)
+            plain = {"seconds": e, "accuracy": a,
+                     "sinkhorn_iterations": sum(p.iteration_count for p in pprof.sinkhorn_profiles),
+                     "sweep_ms": pms, "sweep_launches": pcnt,
+                     "path": "two-sweep Sinkhorn (two reads of C per iteration), FP32 CSR gradient"}
+            log(f"plain two-sweep: {e:.2f}s acc {a:.4f}")
+        else:
+            plain = budget.skipped(2.1 * step_s)
+
+    cpu = None
+    if not args.no_cpu:
+        # FP64 oracle Sinkhorn iterations at n = 16384 take ~0.7 s each
+        secs = min(args.cpu_seconds, budget.left() - 25)
+        if secs >= 2:
+            log("timing the FP64 CPU oracle sample")
+            cpu = cpu_baseline(min(args.n, 16384), args.reg, secs, iters_total)
+            if args.n > 16384:
+                scale = (args.n / 16384) ** 2
+                cpu["value"] *= scale
+                cpu["sample"] += f"; measured at n=16384 and scaled by (n/16384)^2 = {scale:.1f}"
+        else:
+            cpu = budget.skipped(27)
 
     precision = None
-    if not args.no_precision_form:
-        e, a, xprof, _, Px = run_alignment(inst, make_config(True, args, precision="bf16x2"), False, inputs)
-        from cugal_amd.profile import Phase
-
-        grad_s = xprof.phase_times.get(Phase.GRADIENT, 0.0)
-        steps = xprof.frank_wolfe_iterations
+    if not args.no_precision_form and budget.allows("precision-form stage errors", 20):
+        t_mfma = mfma_gradient_seconds(inputs, args.n)
+        est = value - grad_s + fw_steps * t_mfma
         npad = -(-args.n // 256) * 256
-        issued = steps * 8.0 * npad ** 3 / grad_s / 1e12 if grad_s else None
         precision = {
-            "seconds": e, "accuracy": a, "gradient_seconds": grad_s, "frank_wolfe_steps": steps,
-            "mfma_tflops_issued": issued,
-            "mfma_tflops_algorithmic": steps * 4.0 * args.n ** 3 / grad_s / 1e12 if grad_s else None,
-            "mfma_peak_tflops": 2500.0, "mfma_frac": (issued / 2500.0) if issued else None,
-            "gradient_error": gradient_error_sample(inputs[0], inputs[1], Px),
+            "form": "gradient on the matrix cores: exact 0/1 BF16 adjacency times P split into BF16 hi + lo, "
+                    "FP32 accumulation; FP32 log-Sinkhorn; FP64 marginal errors and duality gap",
+            "gradient_seconds_each": t_mfma,
+            "mfma_tflops_issued": 8.0 * npad ** 3 / t_mfma / 1e12,
+            "mfma_tflops_algorithmic": 4.0 * args.n ** 3 / t_mfma / 1e12,
+            "mfma_peak_tflops": 2500.0, "mfma_frac_algorithmic": 4.0 * args.n ** 3 / t_mfma / 1e12 / 2500.0,
+            "estimated_seconds": est,
+            "estimate": f"the headline alignment with its {fw_steps} CSR gradients ({grad_s:.2f} s) replaced by "
+                        f"{fw_steps} timed matrix-core gradients",
+            "estimated_ratio_to_plain": (plain["seconds"] / est) if plain and "seconds" in plain else None,
+            "why_not_faster_here": f"at this graph's density the two CSR gathers need ~8.6e10 reads per gradient "
+                                   f"against 4 n^3 = {4.0 * args.n ** 3:.2e} MFMA flops; the matrix cores win "
+                                   f"above ~1.4 % density",
+            "stage_errors_vs_fp64": stage_errors(inputs, P, args),
         }
-        Px = None
-        log(f"precision form (MFMA gradient): {e:.2f}s acc {a:.4f}, gradient {grad_s:.2f}s "
-            f"({issued:.0f} TFLOP/s issued)")
+        log(f"precision form: MFMA gradient {t_mfma * 1e3:.1f} ms; per-stage errors measured")
+    P = None
 
-    diff = None
+    legs = {}
+    if not args.no_cpu:
+        if budget.allows("reference case", 25):
+            legs["reference_case"] = reference_case()
+            rc = legs["reference_case"]
+            log(f"reference case ER(1000), default Config: device {rc['gpu_seconds']:.2f}s acc "
+                f"{rc['gpu_accuracy']:.4f}; FP64 oracle on {rc['cpu_threads']} threads "
+                f"{rc['cpu_seconds']:.2f}s acc {rc['cpu_accuracy']:.4f}")
+        else:
+            legs["reference_case"] = budget.skipped(25)
     if not args.no_diff:
-        diff = differentiation_form(args.seed)
-        log(f"differentiation form: forward {diff['forward_ms']:.1f} ms, backward {diff['backward_ms']:.1f} ms "
-            f"(x{diff['backward_over_forward']:.2f}); dL/dw rel err {diff['max_rel_err_grad_w']:.2e}")
+        if budget.allows("differentiation form", 20):
+            diff = differentiation_form(args.seed)
+            legs["differentiation_form"] = diff
+            log(f"differentiation form: forward {diff['forward_ms']:.1f} ms, backward {diff['backward_ms']:.1f} ms "
+                f"(x{diff['backward_over_forward']:.2f}); dL/dw rel err {diff['max_rel_err_grad_w']:.2e}")
+        else:
+            legs["differentiation_form"] = budget.skipped(20)
+    if not args.no_extra_workloads:
+        if budget.allows("BA n=8192", 25):
+            legs["ba8192"] = ba8192_leg(args, cpu=False)
+        else:
+            legs["ba8192"] = budget.skipped(25)
+        if budget.allows("SBM n=65536", 60):
+            legs["sbm65536"] = sbm65536_leg(args, precision_form=False)
+        else:
+            legs["sbm65536"] = budget.skipped(60)
+    if precision is not None and not args.no_precision_form:
+        if budget.allows("precision form, whole alignment", 2.8 * step_s + 10):
+            e, a, xprof, _, _ = run_alignment(inst, make_config(True, args, precision="bf16x2"), False, inputs)
+            precision["seconds"] = e
+            precision["accuracy"] = a
+            precision["gradient_seconds"] = xprof.phase_times.get(Phase.GRADIENT, 0.0)
+            precision["ratio_to_plain"] = (plain["seconds"] / e) if plain and "seconds" in plain else None
+            log(f"precision form (MFMA gradient): {e:.2f}s acc {a:.4f}")
+        else:
+            precision["seconds"] = budget.skipped(2.8 * step_s)["skipped"]
 
     n = args.n
     avg_sweep_ms = sweep_ms / max(1, sweep_n)
@@ -544,21 +761,8 @@ def main() -> int:
         "bytes_per_launch": sweep_bytes, "avg_launch_ms": avg_sweep_ms, "launches": sweep_n,
     }
 
-    cpu = None
-    ref_case = None
-    if not args.no_cpu:
-        ref_case = reference_case()
-        log(f"reference case ER(1000), default Config: device {ref_case['gpu_seconds']:.2f}s acc "
-            f"{ref_case['gpu_accuracy']:.4f}; FP64 oracle on {ref_case['cpu_threads']} threads "
-            f"{ref_case['cpu_seconds']:.2f}s acc {ref_case['cpu_accuracy']:.4f}")
-        log("timing the FP64 CPU oracle sample")
-        cpu = cpu_baseline(min(n, 16384), args.reg, args.cpu_seconds, iters_total)
-        if n > 16384:
-            scale = (n / 16384) ** 2
-            cpu["value"] *= scale
-            cpu["sample"] += f"; measured at n=16384 and scaled by (n/16384)^2 = {scale:.1f}"
-
     phase = {k.value: v for k, v in profile.phase_times.items()} if profile else {}
+    order = "natural labels" if args.node_order == "natural" else f"node_order {args.node_order}"
     out = {
         "metric": "align time (s) n=32k mixed vs FP32 plain @ equal node acc; Sinkhorn HBM GB/s",
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -566,14 +770,19 @@ def main() -> int:
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"watts_strogatz n={n} k={args.k} p={args.p}, {args.noise:.0%} edge noise, "
                                f"seeded permutation; LOG reg {args.reg}, thr {args.threshold}, mu {args.mu}, "
-                               f"{args.iter_count}x{args.fw_iter_count} FW, DOUBLE_GREEDY",
+                               f"{args.iter_count}x{args.fw_iter_count} FW, DOUBLE_GREEDY; value = fused Sinkhorn "
+                               f"(one read of C per iteration) with the {form} gradient "
+                               f"({'CSR gathers' if form == 'fp32' else 'matrix cores'}, chosen by "
+                               f"gradient_precision auto at this density), {order}",
                    "n": n, "parallelism": "single GPU"},
         "accuracy": sum(accs) / len(accs), "sinkhorn_iterations": iters_total,
         "sinkhorn_gbs": achieved, "phase_times": phase, "api_extra_s": api_extra, "plain": plain,
-        "speedup_vs_plain": (plain["seconds"] / value) if plain else None,
-        "precision_form": precision, "differentiation_form": diff,
-        "roofline": roofline, "cpu_baseline": cpu, "reference_case": ref_case,
+        "speedup_vs_plain": (plain["seconds"] / value) if plain and "seconds" in plain else None,
+        "precision_form": precision,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
+    out.update(legs)
+    out["wall_s"] = time.time() - T_START
     print(json.dumps(out), flush=True)
     return 0
 
@@ -589,10 +798,11 @@ def _contract_line(value, args, workload: str, n: int, extra: dict) -> str:
     return json.dumps(out)
 
 
-def run_ba8192(args) -> int:
-    """Barabasi-Albert n=8192, m=5, 2% edge noise; the alignment with the
-    exact device Hungarian (DENSE), the same P rounded by scipy's
-    linear_sum_assignment on the host and by the greedy walk."""
+def ba8192_leg(args, cpu: bool) -> dict:
+    """Barabasi-Albert n=8192, m=5, 2% edge noise: the alignment with the
+    exact device Hungarian (DENSE); the same P rounded by scipy's
+    linear_sum_assignment on the host and by the greedy walk; with `cpu`,
+    the FP64 oracle timed per stage and extrapolated."""
     import numpy as np
     import scipy.optimize
     import torch
@@ -603,12 +813,13 @@ def run_ba8192(args) -> int:
     from cugal_amd.synthetic import alignment_accuracy, barabasi_albert
 
     n = 8192
+    ba_args = argparse.Namespace(**{**vars(args), "iter_count": 15, "fw_iter_count": 10})
     inst = barabasi_albert(n, 5, 0.02, seed=args.seed)
     small = barabasi_albert(512, 5, 0.02, seed=1)
-    warm = argparse.Namespace(**{**vars(args), "iter_count": 1, "fw_iter_count": 2})
+    warm = argparse.Namespace(**{**vars(ba_args), "iter_count": 1, "fw_iter_count": 2})
     run_alignment(small, make_config(True, warm, lap="DENSE"), False)
     inputs = device_inputs(inst, "cuda:0")
-    e, acc, prof, (ms, cnt), P = run_alignment(inst, make_config(True, args, lap="DENSE"), True, inputs)
+    e, acc, prof, (ms, cnt), P = run_alignment(inst, make_config(True, ba_args, lap="DENSE"), True, inputs)
     iters = sum(p.iteration_count for p in prof.sinkhorn_profiles)
     col_dense = dense_hungarian(P, maximize=True)
     torch.cuda.synchronize()
@@ -629,8 +840,10 @@ def run_ba8192(args) -> int:
     col_dense = col_dense.cpu().numpy()
     col_greedy = col_greedy.cpu().numpy()
     avg = ms / max(1, cnt)
-    extra = {
-        "accuracy": acc, "sinkhorn_iterations": iters,
+    out = {
+        "workload": f"barabasi_albert n={n} m=5, 2% edge noise, seeded permutation; LOG reg {args.reg}, "
+                    f"15x10 FW, DENSE (device Hungarian)",
+        "seconds": e, "accuracy": acc, "sinkhorn_iterations": iters,
         "phase_times": {k.value: v for k, v in prof.phase_times.items()},
         "lap": {
             "hungarian_device_s": t_dense, "hungarian_free_rows_after_warm_start": free_rows,
@@ -643,20 +856,20 @@ def run_ba8192(args) -> int:
         },
         "sinkhorn_gbs": 4.0 * n * n / (avg * 1e-3) / 1e9 if cnt else None,
     }
+    from cugal_amd.profile import Phase
+
+    out["sinkhorn_us_per_iteration"] = prof.phase_times.get(Phase.SINKHORN, 0.0) / max(1, iters) * 1e6
     log(f"ba8192: {e:.2f}s acc {acc:.4f}; hungarian {t_dense * 1e3:.1f} ms vs scipy {t_scipy * 1e3:.0f} ms, "
-        f"equal={extra['lap']['hungarian_equals_scipy']}")
-    if not args.no_cpu:
+        f"equal={out['lap']['hungarian_equals_scipy']}")
+    if cpu:
         log("timing the FP64 CPU oracle per stage")
-        extra["cpu_baseline"] = cpu_stage_baseline(inst.source, inst.target, args.reg, prof.frank_wolfe_iterations,
-                                                   iters, args.cpu_seconds)
-        log(f"FP64 oracle estimate: {extra['cpu_baseline']['value']:.0f}s ({extra['cpu_baseline']['sample']})")
-    print(_contract_line(e, args, f"barabasi_albert n={n} m=5, 2% edge noise, seeded permutation; LOG reg "
-                                  f"{args.reg}, {args.iter_count}x{args.fw_iter_count} FW, DENSE (device Hungarian)",
-                         n, extra), flush=True)
-    return 0
+        out["cpu_baseline"] = cpu_stage_baseline(inst.source, inst.target, args.reg, prof.frank_wolfe_iterations,
+                                                 iters, args.cpu_seconds)
+        log(f"FP64 oracle estimate: {out['cpu_baseline']['value']:.0f}s ({out['cpu_baseline']['sample']})")
+    return out
 
 
-def run_sbm65536(args) -> int:
+def sbm65536_leg(args, precision_form: bool) -> dict:
     """Stochastic block model: 64 blocks of 1024 nodes, intra/inter degrees
     48/16, 1% edge noise; 15 lambda stages of one Frank-Wolfe step, greedy
     LAP.  P (17 GB) and the cost stay in HBM; the Sinkhorn uses the
@@ -664,23 +877,26 @@ def run_sbm65536(args) -> int:
     import torch
 
     from cugal_amd.profile import Phase
-    from cugal_amd.synthetic import stochastic_block
+    from cugal_amd.synthetic import alignment_accuracy, stochastic_block
 
     t = time.perf_counter()
     inst = stochastic_block(64, 1024, 48.0, 16.0, 0.01, seed=args.seed)
-    n = inst.source.number_of_nodes()
-    log(f"sbm instance n={n} in {time.perf_counter() - t:.1f}s ({inst.source.number_of_edges()} edges)")
+    n = inst.n
+    log(f"sbm instance n={n} in {time.perf_counter() - t:.1f}s ({len(inst.source_edges)} edges)")
     cfg_args = argparse.Namespace(**{**vars(args), "iter_count": 15, "fw_iter_count": 1})
     small = watts_small()
     run_alignment(small, make_config(True, argparse.Namespace(**{**vars(args), "iter_count": 1,
                                                                  "fw_iter_count": 2})), False)
-    inputs = device_inputs(inst, "cuda:0")
+    inputs = inst.adjacencies("cuda:0")
     e, acc, prof, (ms, cnt), P = run_alignment(inst, make_config(True, cfg_args), True, inputs)
     P = None
     avg = ms / max(1, cnt)
     gbs = 4.0 * n * n / (avg * 1e-3) / 1e9 if cnt else None
-    extra = {
-        "accuracy": acc, "sinkhorn_iterations": sum(p.iteration_count for p in prof.sinkhorn_profiles),
+    out = {
+        "workload": f"stochastic_block 64 x 1024, degrees 48/16, 1% edge noise, seeded permutation; LOG reg "
+                    f"{args.reg}, 15 lambda stages x 1 FW step, DOUBLE_GREEDY",
+        "n": n, "seconds": e, "accuracy": acc,
+        "sinkhorn_iterations": sum(p.iteration_count for p in prof.sinkhorn_profiles),
         "phase_times": {k.value: v for k, v in prof.phase_times.items()},
         "sinkhorn_gbs": gbs, "sinkhorn_sweep_avg_ms": avg, "sinkhorn_sweeps": cnt,
         "roofline": {"kernel": "sk_col_sweep / sk_row_sweep (two reads of C per iteration)", "bound": "hbm",
@@ -689,7 +905,7 @@ def run_sbm65536(args) -> int:
     }
     log(f"sbm65536: {e:.2f}s acc {acc:.4f}, sinkhorn sweeps {gbs:.0f} GB/s, "
         f"phases {[(k.value, round(v, 2)) for k, v in prof.phase_times.items()]}")
-    if not args.no_precision_form:
+    if precision_form:
         # The precision form at this scale: the gradient on the matrix cores
         # (0.1 % density, so "auto" above chose the CSR gathers).
         ex, ax, xprof, _, Px = run_alignment(inst, make_config(True, cfg_args, precision="bf16x2"), False, inputs)
@@ -697,7 +913,7 @@ def run_sbm65536(args) -> int:
         steps = xprof.frank_wolfe_iterations
         npad = -(-n // 256) * 256
         issued = steps * 8.0 * npad ** 3 / grad_s / 1e12 if grad_s else None
-        extra["precision_form"] = {
+        out["precision_form"] = {
             "seconds": ex, "accuracy": ax, "gradient_seconds": grad_s, "frank_wolfe_steps": steps,
             "phase_times": {k.value: v for k, v in xprof.phase_times.items()},
             "mfma_tflops_issued": issued,
@@ -707,10 +923,7 @@ def run_sbm65536(args) -> int:
         }
         Px = None
         log(f"sbm65536 precision form: {ex:.2f}s acc {ax:.4f}, gradient {grad_s:.2f}s ({issued:.0f} TFLOP/s issued)")
-    print(_contract_line(e, args, f"stochastic_block 64 x 1024, degrees 48/16, 1% edge noise, seeded "
-                                  f"permutation; LOG reg {args.reg}, 15 lambda stages x 1 FW step, DOUBLE_GREEDY",
-                         n, extra), flush=True)
-    return 0
+    return out
 
 
 def watts_small():
