@@ -79,6 +79,7 @@ _SIGNATURES = {
     "cg_update_workspace": (c_sz, [c_i64]),
     "cg_update_quasi_permutation_log": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
     "cg_update_quasi_permutation_standard": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_update_quasi_permutation_log_static": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_p, c_f, c_f, c_p, c_p]),
     "cg_lap_workspace": (c_sz, [c_i64]),
     "cg_lap_greedy": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_sz, c_p]),
     "cg_lap_greedy_random": (c_int, [c_p, c_i64, c_int, c_p, c_p, c_p, c_p, c_sz, c_p]),

@@ -6,6 +6,8 @@
 //   gap = sum (plan - P) C,  P += a (plan - P)
 // One read of C, one read and one write of P: 12 n^2 bytes.  The gap is
 // accumulated in FP64 and reduced in a fixed order (deterministic).
+#include <mutex>
+
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -14,6 +16,9 @@ namespace {
 
 constexpr int kUpdThreads = 256;
 constexpr int kUpdMaxBlocks = 2048;
+// Largest n of the entry point without a workspace (the same limit as the
+// others: 64-bit offsets, int row indices).
+constexpr int64_t kUpdateStaticMaxN = int64_t(1) << 30;
 
 template <bool STD>
 __device__ __forceinline__ float plan_of(float c, float neg_rinv, float ui, float vj) {
@@ -81,6 +86,35 @@ int update_blocks(int64_t n, int* rows_per_blk) {
     return static_cast<int>(div_up(n, rpb));
 }
 
+// Partial gaps of the entry point that allocates nothing: one double per
+// block of fw_update (at most kUpdMaxBlocks for any n) plus the 256 bytes of
+// slack the workspace size includes, a fixed 16.25 KiB that each device
+// holds for the life of the process.  Host code never passes this name to a
+// kernel; it passes the device address looked up per device.
+constexpr int kScratchDoubles = kUpdMaxBlocks + 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on the current device, looked up once per device.
+int scratch_address(double** out) {
+    int dev = 0;
+    CG_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CG_EUNSUPPORTED, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !ptr)
+            return fail(CG_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(rc));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return CG_OK;
+}
+
 }  // namespace
 }  // namespace cg
 
@@ -134,6 +168,21 @@ int cg_update_quasi_permutation_log(float* P, int64_t ld_p, const float* C, int6
                                     void* workspace, size_t workspace_bytes, cg_stream_t stream) {
     return run_update(false, P, ld_p, C, n, u, v, alpha, regularization, gap_device, workspace, workspace_bytes,
                       reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_log");
+}
+
+int cg_update_quasi_permutation_log_static(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                           const float* v, float alpha, float regularization, double* gap_device,
+                                           cg_stream_t stream) {
+    const char* who = "cg_update_quasi_permutation_log_static";
+    if (n <= 0 || n > kUpdateStaticMaxN)
+        return fail(CG_EINVAL, "%s: n = %lld outside 1..%lld", who, static_cast<long long>(n),
+                    static_cast<long long>(kUpdateStaticMaxN));
+    int rpb = 0;
+    if (update_blocks(n, &rpb) > kUpdMaxBlocks) return fail(CG_EINVAL, "%s: too many partial sums", who);
+    double* part = nullptr;
+    if (int rc = scratch_address(&part)) return rc;
+    return run_update(false, P, ld_p, C, n, u, v, alpha, regularization, gap_device, part,
+                      sizeof(double) * kScratchDoubles, reinterpret_cast<hipStream_t>(stream), who);
 }
 
 int cg_update_quasi_permutation_standard(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,

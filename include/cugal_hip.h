@@ -285,6 +285,17 @@ int cg_update_quasi_permutation_log(float* P, int64_t ld_p, const float* C, int6
                                     const float* v, float alpha, float regularization, double* gap_device,
                                     void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
+/* No counterpart in the reference: cg_update_quasi_permutation_log for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that forbid hipMalloc).  It takes no workspace and allocates nothing: its
+ * partial sums live in a fixed device array of the library (16.25 KiB per
+ * device, held until the process ends; n up to 2^30, larger n is rejected).
+ * Every call on one device shares that array, so calls that overlap on one
+ * device (concurrent streams or threads) race; calls in stream order do not. */
+int cg_update_quasi_permutation_log_static(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
+                                           const float* v, float alpha, float regularization, double* gap_device,
+                                           cg_stream_t stream);
+
 /* The STANDARD form of the same step (cugal/pred.py:77-86 with
  * scale_kernel_matrix, cugal/sinkhorn.py:20-23; gap as :139-141):
  * plan = u_i exp(-C/reg) v_j with the scalings of cg_sinkhorn_standard_solve. */
