@@ -49,6 +49,10 @@ _SIGNATURES = {
     "cg_gradient_workspace": (c_sz, [c_i64]),
     "cg_gradient_padded_ld": (c_i64, [c_i64]),
     "cg_gradient": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_int, c_f, c_p, c_p, c_sz, c_p]),
+    "cg_gradient_directed_workspace": (c_sz, [c_i64]),
+    "cg_gradient_directed": (
+        c_int, [c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_p, c_i64, c_p, c_p, c_int, c_int, c_f, c_p, c_p, c_sz, c_p],
+    ),
     "cg_permute_copy": (c_int, [c_p, c_i64, c_i64, c_p, c_p, c_p, c_i64, c_p]),
     "cg_adjacency_bits_words": (c_sz, [c_i64]),
     "cg_adjacency_pack_bits": (c_int, [c_p, c_p, c_i64, c_p, c_p]),

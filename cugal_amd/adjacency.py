@@ -58,7 +58,13 @@ class Adjacency:
     symmetrised edge list gives it (cugal/adjacency.py:52-53): the loop
     contributes 2 to the SpMM; the 0/1 bit form cannot hold that."""
 
+    directed: bool = False
+    """True when the matrix need not be symmetric (a directed graph, or a
+    dense matrix that is not symmetric): the gradient then takes the
+    general form with the transpose (cugal/pred.py:38-51)."""
+
     _packed_bits: torch.Tensor | None = field(default=None, init=False, repr=False, compare=False)
+    _transposed: "Adjacency | None" = field(default=None, init=False, repr=False, compare=False)
 
     @classmethod
     def from_dense(cls, dense: torch.Tensor) -> "Adjacency":
@@ -67,7 +73,7 @@ class Adjacency:
         rows, cols = torch.nonzero(dense, as_tuple=True)
         edges = torch.stack((cols, rows), dim=1).to(torch.int32)
         col, rp = edges_to_csr(edges, dense.shape[0], symmetrize=False)
-        return cls(col, rp)
+        return cls(col, rp, directed=not bool(torch.equal(dense, dense.T)))
 
     @classmethod
     def from_graph(cls, graph: nx.Graph, device, loops_once: bool = False) -> "Adjacency":
@@ -95,7 +101,27 @@ class Adjacency:
             return cls(col, rp)
         t = torch.from_numpy(edges.astype(np.int32)).to(dev)
         col, rp = edges_to_csr(t, node_count, symmetrize=undirected)
-        return cls(col, rp, loops_twice=undirected and loops > 0)
+        # a directed graph: each edge (x, y) is the entry [y][x], so row y
+        # lists the predecessors of y, as the reference's create_adjacency
+        # gives them (cuda_kernels/adjacency.cu:228-259)
+        return cls(col, rp, loops_twice=undirected and loops > 0, directed=not undirected)
+
+    def transposed(self) -> "Adjacency":
+        """The CSR of the transposed matrix (built once on the device and
+        cached; the adjacency itself for a symmetric one)."""
+        if not self.directed:
+            return self
+        if self._transposed is None:
+            n = self.number_of_nodes()
+            dev = self.row_pointers.device
+            counts = (self.row_pointers[1:] - self.row_pointers[:-1]).to(torch.int64)
+            rows = torch.repeat_interleave(torch.arange(n, device=dev), counts)
+            # the pair (r, c) is the entry [c][r]
+            edges = torch.stack((rows, self.col_indices.to(torch.int64)), dim=1).to(torch.int32)
+            col, rp = edges_to_csr(edges, n, symmetrize=False)
+            self._transposed = Adjacency(col, rp, directed=True)
+            self._transposed._transposed = self
+        return self._transposed
 
     def as_dense(self, dtype: torch.dtype) -> torch.Tensor:
         """Convert back to a dense matrix (format conversion)."""

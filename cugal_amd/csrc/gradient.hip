@@ -157,12 +157,20 @@ __device__ __forceinline__ void gather_range(float4& acc, const int* __restrict_
 // Column indices of a tile's rows are staged in LDS in chunks of this many.
 constexpr int kIdxChunk = 2048;
 
+// A second neighbour list gathered into the same sums (the directed
+// gradient adds B Y1 and B^T Y2 tile by tile); rp == nullptr: none.
+struct SecondList {
+    const int32_t* rp;
+    const int32_t* col;
+    const float* X;
+};
+
 template <bool EPI, bool VEC, int FD, int TC>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
                                                            const float* __restrict__ X, int64_t ldx, float scale,
                                                            Epilogue ep, float* __restrict__ out, int64_t ldo,
-                                                           int tiles_r, int64_t tiles) {
+                                                           int tiles_r, int64_t tiles, SecondList second) {
     constexpr int LPT = TC / 4;              // threads per TC-float segment
     constexpr int RG = kSpThreads / LPT;     // row groups
     constexpr int RPT = kTileR / RG;         // rows per thread
@@ -180,25 +188,30 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         const int g = threadIdx.x / LPT;  // row group
         const int q = threadIdx.x % LPT;  // float4 slot within the segment
         const int cq = c0 + 4 * q;
-        if (threadIdx.x <= kTileR) srp[threadIdx.x] = rp[min(r0 + static_cast<int>(threadIdx.x), n)];
-        __syncthreads();
-        const int e_lo = srp[0];
-        const int e_hi = srp[kTileR];
         float4 acc[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int cs = e_lo; cs < e_hi; cs += kIdxChunk) {
-            const int ce = min(e_hi, cs + kIdxChunk);
-            for (int k = cs + static_cast<int>(threadIdx.x); k < ce; k += kSpThreads) sidx[k - cs] = col[k];
+        for (int list = 0; list < (second.rp ? 2 : 1); ++list) {
+            const int32_t* __restrict__ lrp = list ? second.rp : rp;
+            const int32_t* __restrict__ lcol = list ? second.col : col;
+            const float* __restrict__ lX = list ? second.X : X;
+            if (threadIdx.x <= kTileR) srp[threadIdx.x] = lrp[min(r0 + static_cast<int>(threadIdx.x), n)];
             __syncthreads();
-            if (cq < n) {
+            const int e_lo = srp[0];
+            const int e_hi = srp[kTileR];
+            for (int cs = e_lo; cs < e_hi; cs += kIdxChunk) {
+                const int ce = min(e_hi, cs + kIdxChunk);
+                for (int k = cs + static_cast<int>(threadIdx.x); k < ce; k += kSpThreads) sidx[k - cs] = lcol[k];
+                __syncthreads();
+                if (cq < n) {
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int rr = g + r * RG;
-                    gather_range(acc[r], sidx, cs, max(srp[rr], cs), min(srp[rr + 1], ce), X, ldx, cq);
+                    for (int r = 0; r < RPT; ++r) {
+                        const int rr = g + r * RG;
+                        gather_range(acc[r], sidx, cs, max(srp[rr], cs), min(srp[rr + 1], ce), lX, ldx, cq);
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -215,12 +228,17 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
             const int r = r0 + rr;
             float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
             if (r < n) {
-                for (int k = rp[r]; k < rp[r + 1]; ++k) {
-                    const float* xr = X + static_cast<int64_t>(col[k]) * ldx;
-                    if (cq + 0 < n) acc.x += xr[cq + 0];
-                    if (cq + 1 < n) acc.y += xr[cq + 1];
-                    if (cq + 2 < n) acc.z += xr[cq + 2];
-                    if (cq + 3 < n) acc.w += xr[cq + 3];
+                for (int list = 0; list < (second.rp ? 2 : 1); ++list) {
+                    const int32_t* __restrict__ lrp = list ? second.rp : rp;
+                    const int32_t* __restrict__ lcol = list ? second.col : col;
+                    const float* __restrict__ lX = list ? second.X : X;
+                    for (int k = lrp[r]; k < lrp[r + 1]; ++k) {
+                        const float* xr = lX + static_cast<int64_t>(lcol[k]) * ldx;
+                        if (cq + 0 < n) acc.x += xr[cq + 0];
+                        if (cq + 1 < n) acc.y += xr[cq + 1];
+                        if (cq + 2 < n) acc.z += xr[cq + 2];
+                        if (cq + 3 < n) acc.w += xr[cq + 3];
+                    }
                 }
             }
             tile[rr][4 * q + 0] = acc.x;
@@ -378,9 +396,9 @@ int max_lds_bytes() {
 
 template <bool EPI, int FD, int TC>
 int launch_gather_tile_fd(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
-                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
+                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s, SecondList second) {
     const bool vec = (n % 4 == 0) && (ldx % 4 == 0) && (ldo % 4 == 0) && aligned16(X) && aligned16(out) &&
-                     (!ep.P || aligned16(ep.P)) &&
+                     (!ep.P || aligned16(ep.P)) && (!second.rp || aligned16(second.X)) &&
                      (!ep.fs || (aligned16(ep.fs) && aligned16(ep.ft)));
     const int tiles_r = static_cast<int>(div_up(n, kTileR));
     const int64_t tiles = static_cast<int64_t>(tiles_r) * div_up(n, TC);
@@ -388,23 +406,25 @@ int launch_gather_tile_fd(const int32_t* rp, const int32_t* col, int n, const fl
     if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
     if (vec)
         hipLaunchKernelGGL((gather_tile_t<EPI, true, FD, TC>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
-                           col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
+                           col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles, second);
     else
         hipLaunchKernelGGL((gather_tile_t<EPI, false, FD, TC>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s,
-                           rp, col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
+                           rp, col, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles, second);
     return check_launch("gather_tile_t");
 }
 
 template <bool EPI>
 int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float* X, int64_t ldx, float scale,
-                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s) {
+                       const Epilogue& ep, float* out, int64_t ldo, hipStream_t s,
+                       SecondList second = SecondList{nullptr, nullptr, nullptr}) {
     // The first product (no epilogue) gathers 64-column strips: 256-byte
     // segments, 4 % faster there; the second gathers 32-column strips, whose
     // 4 MiB footprint at n = 32768 already fills an XCD's L2 (64 columns
     // measured 4 % slower).  Same sums in the same order either way.
-    if (!EPI) return launch_gather_tile_fd<EPI, 4, 64>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
-    if (ep.fs && ep.fdim == 8) return launch_gather_tile_fd<EPI, 8, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
-    return launch_gather_tile_fd<EPI, 4, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s);
+    if (!EPI) return launch_gather_tile_fd<EPI, 4, 64>(rp, col, n, X, ldx, scale, ep, out, ldo, s, second);
+    if (ep.fs && ep.fdim == 8)
+        return launch_gather_tile_fd<EPI, 8, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s, second);
+    return launch_gather_tile_fd<EPI, 4, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s, second);
 }
 
 int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float* X, float scale, const Epilogue& ep,
@@ -506,25 +526,32 @@ size_t cg_gradient_workspace(int64_t n) {
 
 int64_t cg_gradient_padded_ld(int64_t n) { return n > 0 ? gradient_ld(n) : 0; }
 
-int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
-                int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
-                int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
-                size_t workspace_bytes, cg_stream_t stream) {
-    if (!a_col || !a_rowptr || !b_col || !b_rowptr || !P || !gradient || bad_n(n) || ld_p < n)
-        return fail(CG_EINVAL, "cg_gradient: bad arguments");
+namespace {
+
+// -2 A P B (symmetric A, B) or -(A P B^T + A^T P B) (directed: the
+// transposes given), + D + factor * lambda (1 - 2P).
+int gradient_impl(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* at_col, const int32_t* at_rowptr,
+                  const int32_t* b_col, const int32_t* b_rowptr, const int32_t* bt_col, const int32_t* bt_rowptr,
+                  int64_t n, const float* P, int64_t ld_p, const float* source_features,
+                  const float* target_features, int feature_dim, int lambda_, float regularizer_factor,
+                  float* gradient, void* workspace, size_t workspace_bytes, hipStream_t s, const char* who) {
+    const bool directed = at_col != nullptr;
+    if (!a_col || !a_rowptr || !b_col || !b_rowptr || !P || !gradient || bad_n(n) || ld_p < n ||
+        (directed && (!at_rowptr || !bt_col || !bt_rowptr)))
+        return fail(CG_EINVAL, "%s: bad arguments", who);
     if ((source_features == nullptr) != (target_features == nullptr))
-        return fail(CG_EINVAL, "cg_gradient: give both feature matrices or neither");
+        return fail(CG_EINVAL, "%s: give both feature matrices or neither", who);
     if (source_features && (!aligned16(source_features) || !aligned16(target_features)))
-        return fail(CG_EINVAL, "cg_gradient: feature matrices must be 16-byte aligned");
-    if (feature_dim != 4 && feature_dim != 8) return fail(CG_EINVAL, "cg_gradient: feature_dim must be 4 or 8");
-    if (!workspace || workspace_bytes < cg_gradient_workspace(n))
-        return fail(CG_EWORKSPACE, "cg_gradient: workspace too small");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        return fail(CG_EINVAL, "%s: feature matrices must be 16-byte aligned", who);
+    if (feature_dim != 4 && feature_dim != 8) return fail(CG_EINVAL, "%s: feature_dim must be 4 or 8", who);
+    const size_t need = directed ? cg_gradient_directed_workspace(n) : cg_gradient_workspace(n);
+    if (!workspace || workspace_bytes < need) return fail(CG_EWORKSPACE, "%s: workspace too small", who);
     const int ni = static_cast<int>(n);
     const int64_t ldy = gradient_ld(n);
     Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
     float* Pp = cv.take<float>(static_cast<size_t>(n) * ldy);
     float* Y = cv.take<float>(static_cast<size_t>(n) * ldy);
+    float* Y2 = directed ? cv.take<float>(static_cast<size_t>(n) * ldy) : nullptr;
     const Epilogue none{nullptr, 0, nullptr, nullptr, 0.0f, 4, 0};
     const float* X1 = P;
     int64_t ld1 = ld_p;
@@ -535,11 +562,44 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
         X1 = Pp;
         ld1 = ldy;
     }
+    // Y = (A P)^T; directed also Y2 = (A^T P)^T
     if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
+    if (directed)
+        if (int rc = launch_gather_tile<false>(at_rowptr, at_col, ni, X1, ld1, 1.0f, none, Y2, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
     Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim,
                 aligned16(P) && ld_p % 4 == 0 ? 1 : 0};
-    return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
+    if (!directed) return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
+    // (B Y + B^T Y2)^T = A P B^T + A^T P B, summed in one tile pass
+    return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -1.0f, ep, gradient, n, s,
+                                    SecondList{bt_rowptr, bt_col, Y2});
+}
+
+}  // namespace
+
+size_t cg_gradient_directed_workspace(int64_t n) {
+    return n > 0 ? 3 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) : 0;
+}
+
+int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
+                int64_t n, const float* P, int64_t ld_p, const float* source_features, const float* target_features,
+                int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
+                size_t workspace_bytes, cg_stream_t stream) {
+    return gradient_impl(a_col, a_rowptr, nullptr, nullptr, b_col, b_rowptr, nullptr, nullptr, n, P, ld_p,
+                         source_features, target_features, feature_dim, lambda_, regularizer_factor, gradient,
+                         workspace, workspace_bytes, reinterpret_cast<hipStream_t>(stream), "cg_gradient");
+}
+
+int cg_gradient_directed(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* at_col,
+                         const int32_t* at_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
+                         const int32_t* bt_col, const int32_t* bt_rowptr, int64_t n, const float* P, int64_t ld_p,
+                         const float* source_features, const float* target_features, int feature_dim, int lambda_,
+                         float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                         cg_stream_t stream) {
+    if (!at_col) return fail(CG_EINVAL, "cg_gradient_directed: the transposes are required");
+    return gradient_impl(a_col, a_rowptr, at_col, at_rowptr, b_col, b_rowptr, bt_col, bt_rowptr, n, P, ld_p,
+                         source_features, target_features, feature_dim, lambda_, regularizer_factor, gradient,
+                         workspace, workspace_bytes, reinterpret_cast<hipStream_t>(stream), "cg_gradient_directed");
 }
 
 }  // extern "C"

@@ -79,7 +79,7 @@ def choose_position(adj: Adjacency, mode: str) -> np.ndarray | None:
     if mode not in ORDERS:
         raise ValueError(f"node_order must be one of {ORDERS}, got {mode!r}")
     n = adj.number_of_nodes()
-    if mode == "natural" or n < 2 or (mode == "auto" and n < AUTO_MIN_NODES):
+    if mode == "natural" or n < 2 or adj.directed or (mode == "auto" and n < AUTO_MIN_NODES):
         return None
     rp, col = _host_csr(adj)
     position = rcm_position(rp, col)

@@ -8,7 +8,9 @@ result with the configured LAP.  Every n x n pass runs in the HIP kernels of
 libcugal_hip.so; the host only sequences them (the SCIPY and SPARSE LAPs run
 scipy on the host, as the reference does).  Supported: config.device a ROCm
 device ("cuda"), dtype torch.float32, SinkhornMethod.LOG or STANDARD,
-undirected graphs.  Anything else raises rather than falling back.
+undirected graphs on either path and directed graphs on the dense path
+(use_sparse_adjacency=False, as the reference).  Anything else raises rather
+than falling back.
 """
 
 from __future__ import annotations
@@ -63,6 +65,12 @@ def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
     enough, else the CSR gathers.  An adjacency that lists a self-loop twice
     (the sparse form of a graph with loops) needs the CSR gathers: the
     matrix-core operand holds 0/1 entries."""
+    if A.directed or B.directed:
+        if precision == "bf16x2":
+            raise NotImplementedError(
+                "gradient_precision='bf16x2' is the symmetric form -2 A P B; a directed graph takes the CSR form"
+            )
+        return "fp32"
     if A.loops_twice or B.loops_twice:
         if precision == "bf16x2":
             raise NotImplementedError(
@@ -79,14 +87,16 @@ def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
 
 def _gradient_into(
     out: torch.Tensor, A: Adjacency, B: Adjacency, P: torch.Tensor, features, iteration: int, factor: float,
-    precision: str = "fp32",
+    precision: str = "fp32", A_transpose: Adjacency | None = None, B_transpose: Adjacency | None = None,
 ) -> torch.Tensor:
-    """out = -2 A P B + D + factor * iteration * (1 - 2P) for symmetric A, B.
-    `features` is a Features (distance recomputed in the kernel), an n x n
-    distance tensor (added afterwards) or None.  precision "fp32": CSR
-    adjacency times FP32 P (cg_gradient); "bf16x2": dense 0/1 BF16
-    adjacency times P split into BF16 hi + lo on the matrix cores, FP32
-    accumulation (cg_gradient_mfma)."""
+    """out = -2 A P B + D + factor * iteration * (1 - 2P) for symmetric A, B;
+    for directed ones (or given transposes) -(A P B^T + A^T P B) + D +
+    factor * iteration * (1 - 2P).  `features` is a Features (distance
+    recomputed in the kernel), an n x n distance tensor (added afterwards)
+    or None.  precision "fp32": CSR adjacency times FP32 P (cg_gradient,
+    cg_gradient_directed); "bf16x2": dense 0/1 BF16 adjacency times P split
+    into BF16 hi + lo on the matrix cores, FP32 accumulation
+    (cg_gradient_mfma, symmetric graphs)."""
     n = P.shape[0]
     dev = P.device
     if P.dim() != 2 or P.stride(1) != 1:
@@ -97,8 +107,22 @@ def _gradient_into(
     if isinstance(features, Features):
         fs, ft = features.source.contiguous(), features.target.contiguous()
         fdim = int(fs.shape[1])
+    general = A.directed or B.directed or (A_transpose is not None and A_transpose is not A) or (
+        B_transpose is not None and B_transpose is not B)
     precision = gradient_precision_for(A, B, precision)
-    if precision == "bf16x2":
+    if general:
+        if precision == "bf16x2":
+            raise NotImplementedError("the matrix-core gradient is the symmetric form -2 A P B")
+        At = A_transpose if A_transpose is not None else A.transposed()
+        Bt = B_transpose if B_transpose is not None else B.transposed()
+        ws = nat.workspace(dev, nat.query("cg_gradient_directed_workspace", n), "gradient")
+        nat.call(
+            "cg_gradient_directed", nat.ptr(A.col_indices), nat.ptr(A.row_pointers), nat.ptr(At.col_indices),
+            nat.ptr(At.row_pointers), nat.ptr(B.col_indices), nat.ptr(B.row_pointers), nat.ptr(Bt.col_indices),
+            nat.ptr(Bt.row_pointers), n, nat.ptr(P), ldp, nat.ptr(fs), nat.ptr(ft), fdim, int(iteration),
+            float(factor), nat.ptr(out), nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+        )
+    elif precision == "bf16x2":
         ws = nat.workspace(dev, nat.query("cg_gradient_mfma_workspace", n), "gradient_mfma")
         nat.call(
             "cg_gradient_mfma", nat.ptr(A.packed_bits()), nat.ptr(B.packed_bits()), n, nat.ptr(P), ldp, nat.ptr(fs),
@@ -126,27 +150,23 @@ def add_feature_distance(gradient: torch.Tensor, features: torch.Tensor | Featur
     return gradient
 
 
-def _is_symmetric(M: torch.Tensor) -> bool:
-    return bool(torch.equal(M, M.T))
-
-
 def dense_gradient(A: torch.Tensor, B: torch.Tensor, P: torch.Tensor, features, iteration: int) -> torch.Tensor:
     """-A^T P B - A P B^T + D + 2 iteration (1 - 2P) for dense adjacency
-    tensors (cugal/pred.py:38-51, whose regulariser is added twice)."""
-    if not (_is_symmetric(A) and _is_symmetric(B)):
-        raise NotImplementedError("the HIP gradient handles undirected graphs (symmetric adjacency)")
+    tensors, symmetric or not (cugal/pred.py:38-51, whose regulariser is
+    added twice); the non-zeros of A and B are taken as CSR on the device."""
     out = torch.empty_like(P)
     return _gradient_into(out, Adjacency.from_dense(A), Adjacency.from_dense(B), P, features, iteration, 2.0)
 
 
 def sparse_gradient(A: Adjacency, B: Adjacency, A_transpose: Adjacency, B_transpose: Adjacency, P: torch.Tensor,
                     features, iteration: int) -> torch.Tensor:
-    """-2 A P B + D + iteration (1 - 2P) for undirected graphs
-    (cugal/pred.py:54-74)."""
-    if not (A is A_transpose and B is B_transpose):
-        raise NotImplementedError("the HIP gradient handles undirected graphs (A is A_transpose)")
+    """-2 A P B + D + iteration (1 - 2P) when A is A_transpose and B is
+    B_transpose, else the general -(A^T P B + A P B^T) + D + iteration
+    (1 - 2P) (cugal/pred.py:54-74)."""
     out = torch.empty_like(P)
-    return _gradient_into(out, A, B, P, features, iteration, 1.0)
+    if A is A_transpose and B is B_transpose:
+        return _gradient_into(out, A, B, P, features, iteration, 1.0)
+    return _gradient_into(out, A, B, P, features, iteration, 1.0, A_transpose=A_transpose, B_transpose=B_transpose)
 
 
 def update_quasi_permutation_log(P: torch.Tensor, K: LogKernel, u: torch.Tensor, v: torch.Tensor, alpha: float,
@@ -202,15 +222,30 @@ def update_quasi_permutation(P: torch.Tensor, K, u: torch.Tensor, v: torch.Tenso
 
 
 def _adjacency_of(graph, config: Config, loops_once: bool = False) -> Adjacency:
+    """The CSR the features are computed from (for a directed graph, row y
+    lists the predecessors of y, as the reference's create_adjacency)."""
     if isinstance(graph, Adjacency):
         return graph
     if isinstance(graph, torch.Tensor):
-        if not _is_symmetric(graph):
-            raise NotImplementedError("the HIP gradient handles undirected graphs (symmetric adjacency)")
         return Adjacency.from_dense(graph.to(config.device))
-    if nx.is_directed(graph):
-        raise NotImplementedError("the HIP gradient handles undirected graphs")
     return Adjacency.from_graph(graph, config.device, loops_once=loops_once)
+
+
+def _gradient_adjacency_of(graph, config: Config) -> Adjacency:
+    """The matrix the gradient multiplies by: nx.to_numpy_array of the graph
+    on the dense path (a loop once; a directed edge x -> y the entry [x][y],
+    cugal/pred.py:106-110), its CSR on the sparse path (undirected only, as
+    the reference asserts at cugal/pred.py:98-105)."""
+    if isinstance(graph, (Adjacency, torch.Tensor)):
+        return _adjacency_of(graph, config)
+    if nx.is_directed(graph):
+        if config.use_sparse_adjacency:
+            raise NotImplementedError(
+                "directed graphs take the dense path (use_sparse_adjacency=False); the reference's sparse path "
+                "asserts undirected graphs (cugal/pred.py:98-105)"
+            )
+        return Adjacency.from_graph(graph, config.device).transposed()
+    return Adjacency.from_graph(graph, config.device, loops_once=not config.use_sparse_adjacency)
 
 
 def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profile) -> torch.Tensor:
@@ -220,8 +255,8 @@ def find_quasi_permutation_matrix(A, B, features, config: Config, profile: Profi
     _check_config(config)
     # Graphs given as networkx: the dense path's adjacency holds a self-loop
     # once (nx.to_numpy_array), the sparse path's CSR twice.
-    A = _adjacency_of(A, config, loops_once=not config.use_sparse_adjacency)
-    B = _adjacency_of(B, config, loops_once=not config.use_sparse_adjacency)
+    A = _gradient_adjacency_of(A, config)
+    B = _gradient_adjacency_of(B, config)
     n = A.number_of_nodes()
     if B.number_of_nodes() != n:
         raise ValueError(f"graphs differ in size ({n} vs {B.number_of_nodes()})")
@@ -355,6 +390,11 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
     nodes of range(max label) they lack, and `profile` defaults to one shared
     instance."""
     _check_config(config)
+    if config.use_sparse_adjacency and (nx.is_directed(source) or nx.is_directed(target)):
+        raise NotImplementedError(
+            "directed graphs take the dense path (use_sparse_adjacency=False); the reference's sparse path asserts "
+            "undirected graphs (cugal/pred.py:98-105)"
+        )
     before = TimeStamp("cpu")
 
     node_count = max(max(source.nodes()), max(target.nodes()))
@@ -366,17 +406,20 @@ def cugal(source: nx.Graph, target: nx.Graph, config: Config, profile: Profile =
         # mismatch inside the gradient)
         raise ValueError(f"graphs differ in size after padding ({source.number_of_nodes()} vs "
                          f"{target.number_of_nodes()} nodes)")
-    if config.node_features == "fugal7" and (nx.number_of_selfloops(source) or nx.number_of_selfloops(target)):
-        raise ValueError("the 7-feature set is defined here for simple graphs (no self-loops)")
+    if config.node_features == "fugal7" and (nx.number_of_selfloops(source) or nx.number_of_selfloops(target)
+                                             or nx.is_directed(source) or nx.is_directed(target)):
+        raise ValueError("the 7-feature set is defined here for simple undirected graphs")
 
     source_adj = _adjacency_of(source, config)
     target_adj = _adjacency_of(target, config)
     gradient_adjacencies = None
-    if not config.use_sparse_adjacency and (source_adj.loops_twice or target_adj.loops_twice):
+    if source_adj.directed or target_adj.directed or (
+            not config.use_sparse_adjacency and (source_adj.loops_twice or target_adj.loops_twice)):
         # The reference's dense gradient multiplies by nx.to_numpy_array (a
-        # loop is 1 on the diagonal); its features come from the CSR (twice).
-        gradient_adjacencies = (_adjacency_of(source, config, loops_once=True),
-                                _adjacency_of(target, config, loops_once=True))
+        # loop is 1 on the diagonal, a directed edge x -> y the entry
+        # [x][y]); its features come from the CSR (a loop twice, a directed
+        # edge in the row of its head).
+        gradient_adjacencies = (_gradient_adjacency_of(source, config), _gradient_adjacency_of(target, config))
 
     quasi_permutation, col_ind = align_device(source_adj, target_adj, config, profile, gradient_adjacencies)
     mapping = [(i, int(col_ind[i])) for i in range(len(quasi_permutation))]

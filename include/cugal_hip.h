@@ -147,6 +147,23 @@ int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_
                 int feature_dim, int lambda_, float regularizer_factor, float* gradient, void* workspace,
                 size_t workspace_bytes, cg_stream_t stream);
 
+/* The gradient of cugal/pred.py:38-51 for directed graphs (any A, B, as
+ * nx.to_numpy_array gives them, cugal/pred.py:106-110), and of the general
+ * four-product branch of sparse_gradient (cugal/pred.py:65-67):
+ *   gradient = -(A P B^T + A^T P B) + D + regularizer_factor * lambda (1 - 2P)
+ * with A, A^T, B, B^T as CSR (row i lists the columns j with M[i][j] != 0,
+ * repeated entries counted).  The two products share their second pass
+ * (one gather of B's and B^T's rows per output tile).  Workspace: the padded
+ * copy of P and the two intermediates (A P)^T, (A^T P)^T, n x (n + 64) FP32
+ * each. */
+size_t cg_gradient_directed_workspace(int64_t n);
+int cg_gradient_directed(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* at_col,
+                         const int32_t* at_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
+                         const int32_t* bt_col, const int32_t* bt_rowptr, int64_t n, const float* P, int64_t ld_p,
+                         const float* source_features, const float* target_features, int feature_dim, int lambda_,
+                         float regularizer_factor, float* gradient, void* workspace, size_t workspace_bytes,
+                         cg_stream_t stream);
+
 /* dst[i][j] = src[row_map[i]][col_map[j]] for n x n FP32 matrices with row
  * strides ld_src, ld_dst (a NULL map is the identity; maps hold values in
  * 0..n-1, which the caller guarantees).  No reference counterpart: the

@@ -58,16 +58,21 @@ def extract_features_nx(G: nx.Graph) -> np.ndarray:
 
 
 def csr_lists(G: nx.Graph) -> list[list[int]]:
-    """The sorted neighbour lists of the reference's CSR for an undirected
-    graph labelled 0..n-1: every edge of nx.edges(G) in both directions, so
-    a self-loop is listed twice (cugal/adjacency.py:46-69).
+    """The sorted neighbour lists of the reference's CSR for a graph
+    labelled 0..n-1 (cugal/adjacency.py:46-69 over create_adjacency,
+    cuda_kernels/adjacency.cu:228-259, which reads each pair (x, y) as the
+    entry [y][x]): an undirected graph lists every edge of nx.edges(G) in
+    both directions, so a self-loop twice; a directed graph lists each edge
+    x -> y once, in row y (the predecessors of y).
     Pin: through extract_features_csr (equal to networkx on simple graphs;
-    the entry-count matrix built from G.edges() on looped graphs)."""
+    the entry-count matrix built from G.edges() on looped or directed
+    graphs)."""
     n = G.number_of_nodes()
     rows = [[] for _ in range(n)]
     for x, y in G.edges():
-        rows[x].append(y)
         rows[y].append(x)
+        if not G.is_directed():
+            rows[x].append(y)
     return [sorted(r) for r in rows]
 
 
@@ -448,10 +453,12 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
     def extract(G):
         if config.node_features != "cugal":
             return extract_features_fugal(G)
-        # Graphs with self-loops: the features of the reference's GPU path
-        # (the path a device run stands in for); the CPU path's networkx
-        # features differ there and agree everywhere else.
-        return extract_features_csr(G) if nx.number_of_selfloops(G) else extract_features_nx(G)
+        # Graphs with self-loops or directed edges: the features of the
+        # reference's GPU path (the path a device run stands in for); the
+        # CPU path's networkx features differ there and agree elsewhere.
+        if nx.number_of_selfloops(G) or G.is_directed():
+            return extract_features_csr(G)
+        return extract_features_nx(G)
 
     fs = torch.tensor(extract(source), dtype=dtype) * config.mu
     ft = torch.tensor(extract(target), dtype=dtype) * config.mu

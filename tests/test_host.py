@@ -122,8 +122,11 @@ def test_input_errors_raise_before_device_work():
     # the padding adds range(max label): label 4 exists only in the source
     with pytest.raises(ValueError, match="differ in size"):
         cugal(nx.path_graph(5), nx.path_graph(3), cfg)
+    # directed graphs take the dense path, as the reference's sparse path
+    # asserts undirected graphs
+    sparse = Config(device="cuda:0", dtype=torch.float32, use_sparse_adjacency=True)
     with pytest.raises(NotImplementedError, match="undirected"):
-        cugal(nx.DiGraph(nx.path_graph(4)), nx.DiGraph(nx.path_graph(4)), cfg)
+        cugal(nx.DiGraph(nx.path_graph(4)), nx.DiGraph(nx.path_graph(4)), sparse)
 
 
 def test_synthetic_noise_free_target_is_permuted_source():

@@ -111,6 +111,55 @@ def test_gpu_path_features_with_self_loops():
     np.testing.assert_array_equal(oracle.extract_features_nx(g)[40], [2.0, 0.0, 0.0, 0.0])
 
 
+@pytest.mark.filterwarnings("ignore:Mean of empty slice:RuntimeWarning")
+@pytest.mark.filterwarnings("ignore:invalid value encountered:RuntimeWarning")
+def test_gpu_path_features_of_a_directed_graph():
+    # The reference's CSR of a directed graph puts edge x -> y in row y
+    # (create_adjacency reads the pair (x, y) as the entry [y][x]), so the
+    # GPU path's features are those of M = A^T (A = nx.to_numpy_array):
+    # degree = in-degree, t(v) = sum_k M[v,k] sum_w min(M[v,w], M[k,w]), and
+    # means over predecessors.
+    g = nx.gnp_random_graph(60, 0.08, seed=5, directed=True)
+    g.add_edge(7, 7)
+    n = g.number_of_nodes()
+    A = nx.to_numpy_array(g, nodelist=range(n)).astype(np.int64)
+    M = A.T
+    t = np.array([sum(M[v, k] * np.minimum(M[v], M[k]).sum() for k in range(n)) for v in range(n)])
+    d = M.sum(1).astype(np.float64)
+    assert np.array_equal(d, [g.in_degree(v) for v in range(n)])
+    clus = np.where((t == 0) | (d < 2), 0.0, t / np.maximum(d * (d - 1), 1))
+    nd = np.where(d > 0, (M @ d) / np.maximum(d, 1), 0.0)
+    nc = np.where(d > 0, (M @ clus) / np.maximum(d, 1), 0.0)
+    f = oracle.extract_features_csr(g)
+    np.testing.assert_array_equal(f[:, 0], d)
+    np.testing.assert_allclose(f[:, 1], clus, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(f[:, 2], nd, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(f[:, 3], nc, rtol=1e-14, atol=1e-15)
+
+
+def test_dense_gradient_of_a_directed_pair_brute_force():
+    # -A^T P B - A P B^T entry by entry from the edge lists (x -> y is
+    # A[x][y]); with P the permutation k -> perm[k] both terms count edge
+    # pairs: (A^T P B)_ij = #{k : k -> i in ga, perm[k] -> j in gb} and
+    # (A P B^T)_ij = #{k : i -> k in ga, j -> perm[k] in gb}.
+    ga = nx.gnp_random_graph(12, 0.3, seed=1, directed=True)
+    gb = nx.gnp_random_graph(12, 0.3, seed=2, directed=True)
+    n = 12
+    A = torch.tensor(nx.to_numpy_array(ga, nodelist=range(n)), dtype=torch.float64)
+    B = torch.tensor(nx.to_numpy_array(gb, nodelist=range(n)), dtype=torch.float64)
+    perm = np.random.default_rng(3).permutation(n)
+    P = torch.zeros((n, n), dtype=torch.float64)
+    P[torch.arange(n), torch.as_tensor(perm)] = 1.0
+    G = oracle.dense_gradient(A, B, P, torch.zeros((n, n), dtype=torch.float64), 0)
+    want = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            t1 = sum(1 for k in range(n) if ga.has_edge(k, i) and gb.has_edge(perm[k], j))
+            t2 = sum(1 for k in range(n) if ga.has_edge(i, k) and gb.has_edge(j, perm[k]))
+            want[i, j] = -(t1 + t2)
+    np.testing.assert_array_equal(G.numpy(), want)
+
+
 def test_adjacency_counts_a_loop_once_dense_twice_sparse():
     g = _loop_graph()
     dense = oracle.adjacency_matrix(g, sparse=False).numpy()
