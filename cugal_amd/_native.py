@@ -70,6 +70,13 @@ _SIGNATURES = {
     "cg_sinkhorn_standard_solve": (
         c_int, [c_p, c_i64, c_f, c_p, c_p, c_int, c_d, c_int, c_int, c_int, c_p, c_int, c_p, c_p, c_p, c_sz, c_p],
     ),
+    "cg_sinkhorn_standard_f64_workspace": (c_sz, [c_i64, c_int, c_int]),
+    "cg_sinkhorn_standard_solve_f64": (
+        c_int, [c_p, c_i64, c_d, c_p, c_p, c_int, c_d, c_int, c_int, c_p, c_int, c_p, c_p, c_p, c_sz, c_p],
+    ),
+    "cg_update_quasi_permutation_f64": (
+        c_int, [c_int, c_p, c_i64, c_p, c_i64, c_p, c_p, c_d, c_d, c_p, c_p, c_sz, c_p],
+    ),
     "cg_feature_cost": (c_int, [c_p, c_p, c_p, c_f, c_i64, c_p, c_p]),
     "cg_sinkhorn_plan": (c_int, [c_p, c_i64, c_f, c_p, c_p, c_p, c_p, c_sz, c_p]),
     "cg_sinkhorn_checkpoint_floats": (c_sz, [c_i64, c_int, c_int]),

@@ -38,8 +38,8 @@ struct SinkCtl {
     int nerr;        // errors recorded so far
     unsigned ticket; // arrival counter of the combine's blocks
     float momentum;  // current over-relaxation factor (1 = plain Sinkhorn)
-    float pad0;
-    int pad1;
+    int pad0;
+    double momentum_d;  // the same factor unrounded (the FP64 solve's)
     // followed by `double errors[capacity]`
 };
 
@@ -64,6 +64,8 @@ __global__ void sk_init(const float* __restrict__ u, const float* __restrict__ v
         ctl->nerr = 0;
         ctl->ticket = 0u;
         ctl->momentum = 1.0f;
+    ctl->momentum_d = 1.0;
+        ctl->momentum_d = 1.0;
     }
 }
 
@@ -382,18 +384,18 @@ __global__ __launch_bounds__(W * kWave) void sk_row_sweep(const float* __restric
 // Lehmann over-relaxation, cugal/sinkhorn.py:78-88, evaluated on the device
 // after `nerr` errors have been recorded.  Python's negative indexing of the
 // error list is kept.
-__device__ float lehmann_momentum(const double* errors, int nerr, int eval_freq, int momentum_start) {
-    if (momentum_start < 0 || nerr * eval_freq < momentum_start) return 1.0f;
+__device__ double lehmann_momentum(const double* errors, int nerr, int eval_freq, int momentum_start) {
+    if (momentum_start < 0 || nerr * eval_freq < momentum_start) return 1.0;
     const int index = momentum_start / eval_freq;
-    if (nerr < index - 1) return 1.0f;
+    if (nerr < index - 1) return 1.0;
     const int a = index - 1 >= 0 ? index - 1 : nerr + index - 1;
     const int b = index - 2 >= 0 ? index - 2 : nerr + index - 2;
-    if (a < 0 || b < 0 || a >= nerr || b >= nerr) return 1.0f;
+    if (a < 0 || b < 0 || a >= nerr || b >= nerr) return 1.0;
     double ratio = errors[a] / errors[b];
     if (isnan(ratio)) ratio = 0.99;
     ratio = fmin(ratio, 0.99);
     const double power = 1.0 / eval_freq;
-    return static_cast<float>(2.0 / (1.0 + sqrt(1.0 - pow(ratio, power))));
+    return 2.0 / (1.0 + sqrt(1.0 - pow(ratio, power)));
 }
 
 // ---------------------------------------------------------------------------
@@ -458,7 +460,8 @@ __device__ void combine_error(double err, int want_err, int check_index, int cur
             ctl->iterations = check_index + 1;
             ctl->final_buf = cur_buf;
         } else {
-            ctl->momentum = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
+            ctl->momentum_d = lehmann_momentum(errors, ctl->nerr, eval_freq, momentum_start);
+            ctl->momentum = static_cast<float>(ctl->momentum_d);
         }
         if (host_flag) __hip_atomic_store(host_flag, sum < threshold ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -746,6 +749,8 @@ __global__ void std_init(const float* __restrict__ u, const float* __restrict__ 
         ctl->nerr = 0;
         ctl->ticket = 0u;
         ctl->momentum = 1.0f;
+    ctl->momentum_d = 1.0;
+        ctl->momentum_d = 1.0;
     }
 }
 
@@ -779,6 +784,130 @@ __global__ void sum_doubles(const double* __restrict__ in, int n, double* __rest
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += in[i];
     s = block_sum_double(s, scratch);
     if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------------------
+// STANDARD Sinkhorn in FP64 (the reference's default dtype, cugal/config.py:
+// 65, for cugal/sinkhorn.py:122-144): the same iteration on an FP64 cost
+// with FP64 scalings, K = exp(-C / reg) evaluated where C is read.  Two
+// sweeps per iteration: the row sweep forms u from v, the column sweep the
+// per-block column sums of the new u, which the combine turns into the next
+// v and the column-marginal error of (u, v), exactly the two-sweep FP32
+// structure.
+// ---------------------------------------------------------------------------
+constexpr double kStdEps64 = 1e-16;  // M_EPS, cugal/sinkhorn.py:12
+
+__device__ __forceinline__ double std_blend64(double old, double fresh, double mom) {
+    return mom == 1.0 ? fresh : pow(old, 1.0 - mom) * pow(fresh, mom);
+}
+
+template <int T, int CPT>
+__global__ __launch_bounds__(T) void std64_col_sweep(const double* __restrict__ C, int n, double nrinv,
+                                                    const double* __restrict__ u, double* __restrict__ part,
+                                                    int rows_per_blk, const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int c0 = blockIdx.y * (T * CPT);
+    double cs[CPT];
+    bool ok[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        ok[k] = c0 + k * T + static_cast<int>(threadIdx.x) < n;
+        cs[k] = 0.0;
+    }
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    for (int i = r0; i < r1; ++i) {
+        const double* row = C + static_cast<int64_t>(i) * n;
+        const double ui = u[i];
+        double c[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) c[k] = ok[k] ? __builtin_nontemporal_load(row + c0 + k * T + threadIdx.x) : 0.0;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+            if (ok[k]) cs[k] = fma(ui, exp(c[k] * nrinv), cs[k]);
+    }
+    double* out = part + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+        if (ok[k]) out[c0 + k * T + threadIdx.x] = cs[k];
+}
+
+template <int W>
+__global__ __launch_bounds__(W * kWave) void std64_row_sweep(const double* __restrict__ C, int n, double nrinv,
+                                                            const double* __restrict__ v, double* __restrict__ u,
+                                                            const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const double mom = ctl->momentum_d;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * W + threadIdx.x / kWave;
+    if (i >= n) return;
+    const double* row = C + i * n;
+    double s = 0.0;
+    for (int j = lane; j < n; j += kWave) s = fma(exp(__builtin_nontemporal_load(row + j) * nrinv), v[j], s);
+    s = wave_sum(s);
+    if (lane == 0) u[i] = std_blend64(u[i], 1.0 / (s + kStdEps64), mom);
+}
+
+__global__ __launch_bounds__(kCombCols* kCombWaves) void std64_combine(
+    const double* __restrict__ part, int nblk, int n, const double* __restrict__ v_in, double* __restrict__ v_out,
+    int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
+    double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
+    __shared__ double rs[kCombWaves][kCombCols];
+    if (ctl->done) return;
+    const int lane = threadIdx.x & (kCombCols - 1);
+    const int wave = threadIdx.x / kCombCols;
+    const int j = blockIdx.x * kCombCols + lane;
+    double sum = 0.0;
+    if (j < n) {
+#pragma unroll 8
+        for (int b = wave; b < nblk; b += kCombWaves) sum += part[static_cast<int64_t>(b) * n + j];
+    }
+    rs[wave][lane] = sum;
+    __syncthreads();
+    double err = 0.0;
+    if (wave == 0 && j < n) {
+        double S = 0.0;
+        for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+        v_out[j] = 1.0 / (S + kStdEps64);
+        if (want_err) err = fabs(v_in[j] * S - 1.0);
+    }
+    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
+}
+
+__global__ void std64_apply_momentum(const double* __restrict__ v_in, double* __restrict__ v_io, int n,
+                                     const SinkCtl* __restrict__ ctl) {
+    if (ctl->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    v_io[j] = std_blend64(v_in[j], v_io[j], ctl->momentum_d);
+}
+
+__global__ void std64_init(const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ uw,
+                           double* __restrict__ vw, int n, SinkCtl* __restrict__ ctl) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        uw[j] = u[j];
+        vw[j] = v[j];
+    }
+    if (j == 0) {
+        ctl->done = 0;
+        ctl->iterations = 0;
+        ctl->final_buf = 0;
+        ctl->nerr = 0;
+        ctl->ticket = 0u;
+        ctl->momentum = 1.0f;
+        ctl->momentum_d = 1.0;
+    }
+}
+
+__global__ void std64_finish(const double* __restrict__ uw, const double* __restrict__ va,
+                             const double* __restrict__ vb, double* __restrict__ u, double* __restrict__ v, int n,
+                             const SinkCtl* __restrict__ ctl, int default_buf) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int buf = ctl->done ? ctl->final_buf : default_buf;
+    u[j] = uw[j];
+    v[j] = buf ? vb[j] : va[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,6 +1242,7 @@ __global__ void sk_ctl_reset(SinkCtl* __restrict__ ctl) {
     ctl->nerr = 0;
     ctl->ticket = 0u;
     ctl->momentum = 1.0f;
+    ctl->momentum_d = 1.0;
 }
 
 }  // namespace
@@ -1472,6 +1602,142 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
     }
     hipLaunchKernelGGL(std_finish, vgrid, dim3(256), 0, s, uw, w.v2[0], w.v2[1], u, v, ni, w.ctl, final_default);
     if (int rc = check_launch("std_finish")) return rc;
+    CG_CHECK_HIP(hipMemcpyAsync(g_mirror.ptr, w.ctl, ctl_bytes(cap), hipMemcpyDeviceToHost, s));
+    CG_CHECK_HIP(hipStreamSynchronize(s));
+    for (auto& pe : pending) (void)hipEventDestroy(pe.ev);
+    const SinkCtl* hc = static_cast<const SinkCtl*>(g_mirror.ptr);
+    const double* herr = reinterpret_cast<const double*>(static_cast<const char*>(g_mirror.ptr) + sizeof(SinkCtl));
+    if (iterations) *iterations = hc->done ? hc->iterations : max_iterations;
+    const int ne = std::min(hc->nerr, errors ? errors_capacity : 0);
+    if (errors) std::memcpy(errors, herr, sizeof(double) * static_cast<size_t>(std::max(0, ne)));
+    if (error_count) *error_count = hc->nerr;
+    return CG_OK;
+}
+
+namespace {
+
+struct SinkWs64 {
+    double* u;
+    double* v[2];
+    double* part;
+    double* errpart;
+    SinkCtl* ctl;
+};
+
+size_t carve64(Carver& cv, int64_t n, const Plan& p, int cap, SinkWs64* w) {
+    SinkWs64 t{};
+    t.u = cv.take<double>(n);
+    t.v[0] = cv.take<double>(n);
+    t.v[1] = cv.take<double>(n);
+    t.part = cv.take<double>(static_cast<size_t>(p.col_nblk) * n);
+    t.errpart = cv.take<double>(p.comb_blocks);
+    t.ctl = reinterpret_cast<SinkCtl*>(cv.take<char>(ctl_bytes(cap)));
+    if (w) *w = t;
+    return cv.used;
+}
+
+}  // namespace
+
+size_t cg_sinkhorn_standard_f64_workspace(int64_t n, int max_iterations, int eval_freq) {
+    if (n <= 0) return 0;
+    Carver cv{nullptr, 0, 0};
+    return carve64(cv, n, make_plan(n, 0), err_capacity(max_iterations, eval_freq), nullptr) + 256;
+}
+
+int cg_sinkhorn_standard_solve_f64(const double* C, int64_t n, double regularization, double* u, double* v,
+                                   int max_iterations, double threshold, int eval_freq, int momentum_start,
+                                   double* errors, int errors_capacity, int* error_count, int* iterations,
+                                   void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!C || !u || !v || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_sinkhorn_standard_solve_f64: bad arguments (n=%lld)", static_cast<long long>(n));
+    if (!(regularization > 0.0)) return fail(CG_EINVAL, "sinkhorn regularization must be > 0");
+    if (eval_freq <= 0) return fail(CG_EINVAL, "sinkhorn eval_freq must be > 0");
+    if (max_iterations < 0) return fail(CG_EINVAL, "sinkhorn iterations must be >= 0");
+    const int ni = static_cast<int>(n);
+    const Plan p = make_plan(n, 0);
+    const int cap = err_capacity(max_iterations, eval_freq);
+    Carver cv{static_cast<char*>(workspace), 0, workspace_bytes};
+    SinkWs64 w{};
+    carve64(cv, n, p, cap, &w);
+    if (!workspace || !cv.ok())
+        return fail(CG_EWORKSPACE, "cg_sinkhorn_standard_solve_f64: workspace too small (%zu < %zu)",
+                    workspace_bytes, cv.used);
+    if (int rc = ensure_pinned(ctl_bytes(cap))) return rc;
+    const double nrinv = -1.0 / regularization;
+    const bool momentum = momentum_start >= 0;
+    const dim3 vgrid(static_cast<unsigned>(div_up(n, 256)));
+    hipLaunchKernelGGL(std64_init, vgrid, dim3(256), 0, s, u, v, w.u, w.v[0], ni, w.ctl);
+    if (int rc = check_launch("std64_init")) return rc;
+
+    struct Pending {
+        hipEvent_t ev;
+        int slot;
+    };
+    std::vector<Pending> pending;
+    int next_slot = 0;
+    bool stopped = false;
+    int check_slot = 0;
+    auto post_check = [&]() -> int {
+        hipEvent_t ev;
+        CG_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const int slot = check_slot;
+        CG_CHECK_HIP(hipEventRecord(ev, s));
+        pending.push_back({ev, slot});
+        if (pending.size() >= 2) {
+            Pending old = pending.front();
+            pending.erase(pending.begin());
+            CG_CHECK_HIP(hipEventSynchronize(old.ev));
+            CG_CHECK_HIP(hipEventDestroy(old.ev));
+            if (g_done_slots[old.slot]) stopped = true;
+        }
+        return 0;
+    };
+    auto col = [&]() -> int {
+        hipLaunchKernelGGL((std64_col_sweep<kColThreads, kColCPT>), dim3(p.col_nblk, p.col_chunks),
+                           dim3(kColThreads), 0, s, C, ni, nrinv, w.u, w.part, p.col_rows, w.ctl);
+        return check_launch("std64_col_sweep");
+    };
+    auto combine = [&](int cur, int want_err, int check_index) -> int {
+        int* flag = nullptr;
+        if (want_err) {
+            check_slot = next_slot++ & 63;
+            g_done_slots[check_slot] = 0;
+            flag = g_done_slots + check_slot;
+        }
+        hipLaunchKernelGGL(std64_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+                           p.col_nblk, ni, w.v[cur], w.v[cur ^ 1], want_err, check_index, cur, threshold, eval_freq,
+                           momentum_start, w.errpart, w.ctl, flag);
+        if (int rc = check_launch("std64_combine")) return rc;
+        if (momentum) {
+            hipLaunchKernelGGL(std64_apply_momentum, vgrid, dim3(256), 0, s, w.v[cur], w.v[cur ^ 1], ni, w.ctl);
+            if (int rc = check_launch("std64_apply_momentum")) return rc;
+        }
+        return 0;
+    };
+    int cur = 0;
+    int final_default = 0;
+    if (max_iterations > 0) {
+        // v from the incoming u, then per iteration: u from v (row sweep),
+        // the column sums of the new u (check of this iteration, next v).
+        if (int rc = col()) return rc;
+        if (int rc = combine(0, 0, -1)) return rc;
+        cur = 1;
+        for (int it = 0; it < max_iterations && !stopped; ++it) {
+            const bool chk = it % eval_freq == 0;  // the reference checks at 0 too (cugal/sinkhorn.py:138)
+            hipLaunchKernelGGL((std64_row_sweep<kRowWaves>), dim3(static_cast<unsigned>(div_up(n, kRowWaves))),
+                               dim3(kRowWaves * kWave), 0, s, C, ni, nrinv, w.v[cur], w.u, w.ctl);
+            if (int rc = check_launch("std64_row_sweep")) return rc;
+            if (int rc = col()) return rc;
+            if (int rc = combine(cur, chk ? 1 : 0, it)) return rc;
+            final_default = cur;
+            cur ^= 1;
+            if (chk)
+                if (int rc = post_check()) return rc;
+        }
+    }
+    hipLaunchKernelGGL(std64_finish, vgrid, dim3(256), 0, s, w.u, w.v[0], w.v[1], u, v, ni, w.ctl, final_default);
+    if (int rc = check_launch("std64_finish")) return rc;
     CG_CHECK_HIP(hipMemcpyAsync(g_mirror.ptr, w.ctl, ctl_bytes(cap), hipMemcpyDeviceToHost, s));
     CG_CHECK_HIP(hipStreamSynchronize(s));
     for (auto& pe : pending) (void)hipEventDestroy(pe.ev);

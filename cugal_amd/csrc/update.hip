@@ -72,6 +72,35 @@ __global__ __launch_bounds__(kUpdThreads) void fw_update(float* __restrict__ P, 
     if (threadIdx.x == 0) part[blockIdx.x] = gap;
 }
 
+// The same step in FP64 (the reference's default dtype): plan, gap and P
+// all in double.
+template <bool STD>
+__global__ __launch_bounds__(kUpdThreads) void fw_update64(double* __restrict__ P, int64_t ldp,
+                                                          const double* __restrict__ C, int n,
+                                                          const double* __restrict__ u, const double* __restrict__ v,
+                                                          double alpha, double nrinv, int rows_per_blk,
+                                                          double* __restrict__ part) {
+    __shared__ double scratch[kUpdThreads / kWave];
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    double gap = 0.0;
+    for (int i = r0; i < r1; ++i) {
+        const double* __restrict__ crow = C + static_cast<int64_t>(i) * n;
+        double* __restrict__ prow = P + static_cast<int64_t>(i) * ldp;
+        const double ui = u[i];
+        for (int j = threadIdx.x; j < n; j += kUpdThreads) {
+            const double c = __builtin_nontemporal_load(crow + j);
+            const double p = prow[j];
+            const double plan = STD ? ui * exp(c * nrinv) * v[j] : exp(fma(c, nrinv, ui) + v[j]);
+            const double d = plan - p;
+            gap += d * c;
+            prow[j] = fma(alpha, d, p);
+        }
+    }
+    gap = block_sum_double(gap, scratch);
+    if (threadIdx.x == 0) part[blockIdx.x] = gap;
+}
+
 __global__ void sum_parts(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double scratch[4];
     double s = 0.0;
@@ -183,6 +212,39 @@ int cg_update_quasi_permutation_log_static(float* P, int64_t ld_p, const float* 
     if (int rc = scratch_address(&part)) return rc;
     return run_update(false, P, ld_p, C, n, u, v, alpha, regularization, gap_device, part,
                       sizeof(double) * kScratchDoubles, reinterpret_cast<hipStream_t>(stream), who);
+}
+
+namespace {
+
+int run_update64(bool standard, double* P, int64_t ldp, const double* C, int64_t n, const double* u,
+                 const double* v, double alpha, double regularization, double* gap_device, void* workspace,
+                 size_t workspace_bytes, hipStream_t s, const char* who) {
+    if (!P || !C || !u || !v || !gap_device || n <= 0 || n > (int64_t(1) << 30) || ldp < n)
+        return fail(CG_EINVAL, "%s: bad arguments", who);
+    if (!(regularization > 0.0)) return fail(CG_EINVAL, "regularization must be > 0");
+    if (!workspace || workspace_bytes < cg_update_workspace(n)) return fail(CG_EWORKSPACE, "%s: workspace too small", who);
+    int rpb = 0;
+    const int blocks = update_blocks(n, &rpb);
+    double* part = static_cast<double*>(workspace);
+    const double nrinv = -1.0 / regularization;
+    if (standard)
+        hipLaunchKernelGGL(fw_update64<true>, dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, static_cast<int>(n),
+                           u, v, alpha, nrinv, rpb, part);
+    else
+        hipLaunchKernelGGL(fw_update64<false>, dim3(blocks), dim3(kUpdThreads), 0, s, P, ldp, C, static_cast<int>(n),
+                           u, v, alpha, nrinv, rpb, part);
+    if (int rc = check_launch("fw_update64")) return rc;
+    hipLaunchKernelGGL(sum_parts, dim3(1), dim3(256), 0, s, part, blocks, gap_device);
+    return check_launch("sum_parts");
+}
+
+}  // namespace
+
+int cg_update_quasi_permutation_f64(int standard, double* P, int64_t ld_p, const double* C, int64_t n,
+                                    const double* u, const double* v, double alpha, double regularization,
+                                    double* gap_device, void* workspace, size_t workspace_bytes, cg_stream_t stream) {
+    return run_update64(standard != 0, P, ld_p, C, n, u, v, alpha, regularization, gap_device, workspace,
+                        workspace_bytes, reinterpret_cast<hipStream_t>(stream), "cg_update_quasi_permutation_f64");
 }
 
 int cg_update_quasi_permutation_standard(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,

@@ -27,8 +27,13 @@ M_EPS = 1e-16
 
 
 def can_use_cuda(config: Config) -> bool:
-    """True when the HIP Sinkhorn applies (cugal/sinkhorn.py:16-17)."""
-    return "cuda" in config.device and config.dtype == torch.float32
+    """True when the HIP Sinkhorn applies (cugal/sinkhorn.py:16-17): FP32
+    for both methods, FP64 for STANDARD."""
+    if "cuda" not in config.device:
+        return False
+    if config.dtype == torch.float32:
+        return True
+    return config.dtype == torch.float64 and config.sinkhorn_method == SinkhornMethod.STANDARD
 
 
 @dataclass
@@ -142,7 +147,7 @@ class SinkhornState:
     def solve_log(self, C: torch.Tensor, config: Config, profile: SinkhornProfile | None = None):
         """Log-domain solve of cugal/sinkhorn.py:90-120 on the device."""
         profile = SinkhornProfile() if profile is None else profile
-        if not can_use_cuda(config):
+        if "cuda" not in config.device or config.dtype != torch.float32:
             raise NotImplementedError("the HIP log-Sinkhorn needs config.device 'cuda' and dtype torch.float32")
         C = C.contiguous()
         n = C.shape[0]
@@ -161,17 +166,21 @@ class SinkhornState:
         return LogKernel(C, config.sinkhorn_regularization), self.u, self.v
 
     def solve_standard(self, C: torch.Tensor, config: Config, profile: SinkhornProfile | None = None):
-        """Scaling-form solve of cugal/sinkhorn.py:122-144 on the device."""
+        """Scaling-form solve of cugal/sinkhorn.py:122-144 on the device, in
+        FP32 or (config.dtype torch.float64, the reference's default) FP64."""
         profile = SinkhornProfile() if profile is None else profile
         if not can_use_cuda(config):
-            raise NotImplementedError("the HIP Sinkhorn needs config.device 'cuda' and dtype torch.float32")
+            raise NotImplementedError("the HIP Sinkhorn needs config.device 'cuda' and dtype torch.float32 "
+                                      "(or torch.float64 for STANDARD)")
         C = C.contiguous()
         n = C.shape[0]
-        nat.require_device_tensor(C, "cost", torch.float32, n * n)
-        nat.require_device_tensor(self.u, "u", torch.float32, n)
-        nat.require_device_tensor(self.v, "v", torch.float32, n)
+        dtype = config.dtype
+        nat.require_device_tensor(C, "cost", dtype, n * n)
+        nat.require_device_tensor(self.u, "u", dtype, n)
+        nat.require_device_tensor(self.v, "v", dtype, n)
         start_time = TimeStamp(config.device)
-        errors, iterations = solve_standard_native(
+        solve = solve_standard_native_f64 if dtype == torch.float64 else solve_standard_native
+        errors, iterations = solve(
             C, config.sinkhorn_regularization, self.u, self.v, config.sinkhorn_iterations,
             config.sinkhorn_threshold, config.sinkhorn_eval_freq, config.sinkhorn_momentum_start,
             fused=config.sinkhorn_fused,
@@ -254,6 +263,31 @@ def solve_standard_native(
         "cg_sinkhorn_standard_solve", nat.ptr(C), n, float(regularization), nat.ptr(u), nat.ptr(v),
         int(max_iterations), float(threshold), int(eval_freq),
         -1 if momentum_start is None else int(momentum_start), int(bool(fused)),
+        errors.ctypes.data, cap, err_count.ctypes.data, iters.ctypes.data,
+        nat.ptr(ws), ws.numel(), nat.stream_of(dev),
+    )
+    count = int(min(err_count[0], cap))
+    return [float(e) for e in errors[:count]], int(iters[0])
+
+
+def solve_standard_native_f64(
+    C: torch.Tensor, regularization: float, u: torch.Tensor, v: torch.Tensor, max_iterations: int,
+    threshold: float, eval_freq: int, momentum_start, fused: bool = False,
+) -> tuple[list[float], int]:
+    """Run cg_sinkhorn_standard_solve_f64 in place on FP64 (u, v); `fused`
+    is accepted for symmetry (the FP64 solve always takes two sweeps).
+    Returns (errors, iteration count)."""
+    n = C.shape[0]
+    dev = C.device
+    ws = nat.workspace(dev, nat.query("cg_sinkhorn_standard_f64_workspace", n, max_iterations, eval_freq), "sinkhorn")
+    cap = max_iterations // max(1, eval_freq) + 4
+    errors = np.zeros(cap, dtype=np.float64)
+    err_count = np.zeros(1, dtype=np.int32)
+    iters = np.zeros(1, dtype=np.int32)
+    nat.call(
+        "cg_sinkhorn_standard_solve_f64", nat.ptr(C), n, float(regularization), nat.ptr(u), nat.ptr(v),
+        int(max_iterations), float(threshold), int(eval_freq),
+        -1 if momentum_start is None else int(momentum_start),
         errors.ctypes.data, cap, err_count.ctypes.data, iters.ctypes.data,
         nat.ptr(ws), ws.numel(), nat.stream_of(dev),
     )

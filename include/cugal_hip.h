@@ -184,6 +184,20 @@ int cg_permute_copy(const float* src, int64_t ld_src, int64_t n, const int32_t* 
 int cg_sinkhorn_log_step(const float* K, int64_t n, const float* add, float* out, float momentum,
                          cg_stream_t stream);
 
+/* Replaces SinkhornState.solve_standard (cugal/sinkhorn.py:122-144) in the
+ * reference's default FP64 (cugal/config.py:65): C, u, v FP64; K =
+ * exp(-C / reg) evaluated where C is read; per iteration v = v^(1-m)
+ * (1 / (u^T K + 1e-16))^m, u = u^(1-m) (1 / (K v + 1e-16))^m, the
+ * column-marginal error sum_j |sum_i u_i K_ij v_j - 1| at iterations 0, f,
+ * 2f, ... and the stop below `threshold`; Lehmann momentum from
+ * `momentum_start` (-1: off), all decided on the device.  Arguments and
+ * outputs as cg_sinkhorn_standard_solve. */
+size_t cg_sinkhorn_standard_f64_workspace(int64_t n, int max_iterations, int eval_freq);
+int cg_sinkhorn_standard_solve_f64(const double* C, int64_t n, double regularization, double* u, double* v,
+                                   int max_iterations, double threshold, int eval_freq, int momentum_start,
+                                   double* errors, int errors_capacity, int* error_count, int* iterations,
+                                   void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
 /* Replaces sinkhorn_log_marginal (cuda_kernels/sinkhorn_log.cu:88-98, bound
  * at cuda_kernels/kernels.cpp:54-57): sum_r |sum_c exp(K[r][c] + v[c] + u[r])
  * - 1| accumulated in FP64; synchronises the stream and writes *result on the
@@ -319,6 +333,14 @@ int cg_update_quasi_permutation_log_static(float* P, int64_t ld_p, const float* 
 int cg_update_quasi_permutation_standard(float* P, int64_t ld_p, const float* C, int64_t n, const float* u,
                                          const float* v, float alpha, float regularization, double* gap_device,
                                          void* workspace, size_t workspace_bytes, cg_stream_t stream);
+
+/* The update in FP64 (the reference's default dtype, cugal/config.py:65):
+ * standard != 0 takes the scalings of cg_sinkhorn_standard_solve_f64
+ * (plan = u_i exp(-C/reg) v_j), else FP64 log duals (plan = exp(-C/reg +
+ * u_i + v_j)); P, C, u, v FP64, the same workspace as the FP32 forms. */
+int cg_update_quasi_permutation_f64(int standard, double* P, int64_t ld_p, const double* C, int64_t n,
+                                    const double* u, const double* v, double alpha, double regularization,
+                                    double* gap_device, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * Rounding to a permutation (LAP).

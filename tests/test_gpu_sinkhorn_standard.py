@@ -171,3 +171,86 @@ def test_pipeline_standard_matches_oracle(device, sparse):
     assert abs(acc - ref_acc) <= 0.005 + 1e-12, (acc, ref_acc)
     got_iters = [p.iteration_count for p in profile.sinkhorn_profiles]
     assert all(abs(a - b) <= 10 for a, b in zip(got_iters, run.sinkhorn_iterations))
+
+
+# ---- FP64 (the reference's default dtype) ---------------------------------
+
+def _run_gpu64(C64, reg, iters, thr, freq, device, momentum_start=None):
+    from cugal_amd.sinkhorn import solve_standard_native_f64
+
+    n = C64.shape[0]
+    C = C64.to(device).contiguous()
+    u = torch.full((n,), 1.0 / n, dtype=torch.float64, device=device)
+    v = torch.full((n,), 1.0 / n, dtype=torch.float64, device=device)
+    errors, count = solve_standard_native_f64(C, reg, u, v, iters, thr, freq, momentum_start)
+    torch.cuda.synchronize()
+    return u.cpu(), v.cpu(), errors, count
+
+
+@pytest.mark.parametrize("n", [64, 333, 1000, 4096])
+def test_fp64_fixed_iterations_match_oracle(device, n):
+    # FP64 against FP64: the same iteration, sums in a different order, so
+    # the scalings agree to a few ulps per iteration (held to 1e-11).
+    C = _cost(n, seed=100 + n)
+    u, v, errors, count = _run_gpu64(C, 1.0, 40, 0.0, 10, device)
+    _, uo, vo, res = oracle.SinkhornState(n, "STANDARD").solve_standard(C.clone(), 1.0, 40, 0.0, 10)
+    assert count == 40 == res.iteration_count
+    np.testing.assert_allclose(u.numpy(), uo.numpy(), rtol=1e-11)
+    np.testing.assert_allclose(v.numpy(), vo.numpy(), rtol=1e-11)
+    np.testing.assert_allclose(errors, res.errors, rtol=1e-8, atol=1e-13 * n)
+
+
+def test_fp64_threshold_stop_and_momentum_match_oracle(device):
+    n = 2000
+    C = _cost(n, seed=7, scale=8.0)
+    for momentum_start in (None, 40):
+        u, v, errors, count = _run_gpu64(C, 0.5, 500, 1e-9, 10, device, momentum_start)
+        st = oracle.SinkhornState(n, "STANDARD")
+        _, uo, vo, res = st.solve_standard(C.clone(), 0.5, 500, 1e-9, 10, momentum_start)
+        print(f"FP64 STANDARD momentum_start={momentum_start}: {count} iterations (oracle {res.iteration_count}), "
+              f"last error {errors[-1]:.3e} (oracle {res.errors[-1]:.3e})")
+        assert count == res.iteration_count < 500
+        assert len(errors) == len(res.errors)
+        np.testing.assert_allclose(errors, res.errors, rtol=1e-6, atol=1e-14 * n)
+        np.testing.assert_allclose(u.numpy(), uo.numpy(), rtol=1e-9)
+        np.testing.assert_allclose(v.numpy(), vo.numpy(), rtol=1e-9)
+
+
+def test_fp64_small_regularization_beyond_fp32_range(device):
+    # reg 0.01 with costs up to 5: exp(-C/reg) reaches e^-500 ~ 1e-217, far
+    # below FP32's range (every FP32 entry of K past C/reg ~ 104 is 0) and
+    # inside FP64's; the FP64 solve follows the oracle there.
+    n = 300
+    C = _cost(n, seed=9, scale=5.0)
+    u, v, errors, count = _run_gpu64(C, 0.01, 60, 0.0, 10, device)
+    _, uo, vo, res = oracle.SinkhornState(n, "STANDARD").solve_standard(C.clone(), 0.01, 60, 0.0, 10)
+    plan = u[:, None] * torch.exp(C / -0.01) * v[None, :]
+    plan_o = uo[:, None] * torch.exp(C / -0.01) * vo[None, :]
+    assert torch.isfinite(plan).all()
+    np.testing.assert_allclose(plan.numpy(), plan_o.numpy(), rtol=1e-8, atol=1e-300)
+    np.testing.assert_allclose(errors, res.errors, rtol=1e-6)
+
+
+def test_fp64_update_matches_oracle(device):
+    from cugal_amd import _native as nat
+
+    n, reg, alpha = 500, 0.7, 0.4
+    C = _cost(n, seed=11)
+    g = torch.Generator().manual_seed(12)
+    u = torch.rand(n, generator=g, dtype=torch.float64) / n
+    v = torch.rand(n, generator=g, dtype=torch.float64) * 3
+    P0 = torch.rand((n, n), generator=g, dtype=torch.float64) / n
+    Cd, ud, vd = C.to(device), u.to(device), v.to(device)  # held across the call
+    for standard in (1, 0):
+        P = P0.to(device)
+        gap = torch.empty(1, dtype=torch.float64, device=device)
+        ws = nat.workspace(device, nat.query("cg_update_workspace", n), "update")
+        nat.call("cg_update_quasi_permutation_f64", standard, nat.ptr(P), n, nat.ptr(Cd), n, nat.ptr(ud),
+                 nat.ptr(vd), alpha, reg, nat.ptr(gap), nat.ptr(ws), ws.numel(), nat.stream_of(device))
+        K = torch.exp(C / -reg)
+        plan = oracle.plan_standard(K, u, v) if standard else torch.exp(C / -reg + u[:, None] + v[None, :])
+        Q = P0.clone()
+        step = oracle.update_quasi_permutation(Q, plan, alpha)
+        np.testing.assert_allclose(P.cpu().numpy(), Q.numpy(), rtol=1e-13, atol=1e-18)
+        want_gap = float((C * (step / alpha)).sum())
+        assert float(gap) == pytest.approx(want_gap, rel=1e-10)
