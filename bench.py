@@ -379,12 +379,24 @@ def probe(args) -> int:
     from cugal_amd.sinkhorn import LogKernel, StandardKernel, solve_log_native, solve_standard_native
     from cugal_amd.synthetic import watts_strogatz
 
+    from cugal_amd import ordering
+    from cugal_amd.sinkhorn import solve_standard_native_f64
+
     n = args.n
     inst = watts_strogatz(n, args.k, args.p, args.noise, seed=args.seed)
     dev = torch.device("cuda:0")
     A = Adjacency.from_graph(inst.source, dev)
     B = Adjacency.from_graph(inst.target, dev)
     feats = Features(node_features(A, args.mu), node_features(B, args.mu))
+    # the CSR gradient on the labels the Frank-Wolfe loop uses
+    # (node_order "auto": the target relabelled by reverse Cuthill-McKee)
+    pb = ordering.choose_position(B, "auto")
+    if pb is not None:
+        pos = torch.from_numpy(pb).to(dev)
+        Bo = ordering.relabel(B, pos)
+        feats_o = Features(feats.source, ordering.permute_rows(feats.target, pos))
+    else:
+        Bo, feats_o = B, feats
     g = torch.Generator(device=dev).manual_seed(0)
     # P row-padded, as the alignment keeps it (the gradient gathers it in place)
     P = torch.empty((n, int(nat.query("cg_gradient_padded_ld", n))), device=dev)[:, :n]
@@ -394,13 +406,25 @@ def probe(args) -> int:
     for lam in (3, 4):
         _gradient_into(G, A, B, P, feats, lam, 1.0, precision="bf16x2")
     for lam in (3, 4):
-        _gradient_into(G, A, B, P, feats, lam, 1.0)
+        _gradient_into(G, A, Bo, P, feats_o, lam, 1.0)
+    # the general (directed) form on the same matrices, as the dense path of
+    # a directed pair runs it
+    Ad = Adjacency(A.col_indices, A.row_pointers, directed=True)
+    Bd = Adjacency(B.col_indices, B.row_pointers, directed=True)
+    _gradient_into(G, Ad, Bd, P, feats, 3, 2.0)
     us = torch.full((n,), 1.0 / n, device=dev)
     vs = torch.full((n,), 1.0 / n, device=dev)
     solve_standard_native(G, 4.0 * args.reg, us, vs, 10, 0.0, 10, None, fused=True)
     u = torch.zeros(n, device=dev)
     v = torch.zeros(n, device=dev)
     solve_log_native(G, args.reg, u, v, 20, 0.0, 10, None, fused=True)
+    # FP64 STANDARD (the reference's default dtype) on an n/2 slice
+    h = n // 2
+    G64 = G[:h, :h].double().contiguous()
+    u64 = torch.full((h,), 1.0 / h, dtype=torch.float64, device=dev)
+    v64 = torch.full((h,), 1.0 / h, dtype=torch.float64, device=dev)
+    solve_standard_native_f64(G64, 4.0 * args.reg, u64, v64, 5, 0.0, 10, None)
+    G64 = u64 = v64 = None
     solve_log_native(G, args.reg, u, v, 10, 0.0, 10, None, fused=False)
     gap = torch.empty(1, dtype=torch.float64, device=dev)
     Q = P.clone()
