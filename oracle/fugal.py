@@ -435,6 +435,8 @@ class OracleRun:
     mapping: list
     sinkhorn_iterations: list
     duality_gaps: list
+    sinkhorn_errors: list = field(default_factory=list)  # per solve, the checked marginal errors
+    cost_absmax: list = field(default_factory=list)  # per solve, max |C|
 
 
 def pad_graphs(source: nx.Graph, target: nx.Graph):
@@ -468,7 +470,7 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
     n = A.shape[0]
     state = SinkhornState(n, config.sinkhorn_method, dtype)
     P = torch.full((n, n), 1 / n, dtype=dtype)
-    iterations, gaps = [], []
+    iterations, gaps, errors, cost_absmax = [], [], [], []
     for lam in range(config.iter_count):
         for it in range(1, config.frank_wolfe_iter_count + 1):
             if config.use_sparse_adjacency:
@@ -486,6 +488,8 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
                                                     config.sinkhorn_momentum_start)
                 plan = plan_standard(K, u, v)
             iterations.append(res.iteration_count)
+            errors.append(list(res.errors))
+            cost_absmax.append(float(G.abs().max()))
             alpha = 2.0 / float(2.0 + it)
             step = update_quasi_permutation(P, plan, alpha)
             gap = duality_gap(G, step, alpha)
@@ -498,7 +502,7 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
     else:
         col = list(scipy_lap(Q))
     mapping = [(i, int(col[i])) for i in range(n)]
-    return OracleRun(Q, mapping, iterations, gaps)
+    return OracleRun(Q, mapping, iterations, gaps, errors, cost_absmax)
 
 
 # ---------------------------------------------------------------------------

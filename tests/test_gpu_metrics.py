@@ -119,51 +119,120 @@ def test_experiment_runs_end_to_end(device, tmp_path):
     assert back.to_dict() == res.to_dict()
 
 
-def test_harness_on_a_reference_dataset_matches_oracle(device):
-    # ca-netscience (379 nodes, 914 edges; the reference's benchmark/data
-    # file, copied to tests/golden) with the configuration and the first
-    # noise level of the reference's published run (CUGAL_res.json: LOG,
-    # reg 0.3, <= 500 iterations, thr 1e-3, mu 2, 15 x 10 FW, sparse, SCIPY;
-    # 10 % of source edges removed; seed 283841238).  The harness on the
-    # device against the FP64 oracle pipeline on the same seeded instance,
-    # within 2 pp of node accuracy and of each structure score, not 0.5:
-    # this instance is ill-conditioned for the oracle itself.  torch's CPU
-    # reductions split by thread count, and the FP64 oracle moves with them:
-    # 1 or 8 threads 30163 Sinkhorn iterations and accuracy 0.6201, 16
-    # threads 26879 and 0.6095 on one machine, 0.6016 on another (threshold
-    # crossings shift, and many structurally equivalent nodes have nearly
-    # equal rows of Q that the LAP orders by rounding).  The device gave
-    # 0.6069.  (The published 0.642 averages four instances drawn after
-    # three other graphs, from a build with the LOG marginal-check quirk; it
-    # is context, not a bound.)
+def _published_config(device_cfg: bool, **kw):
+    """The reference's published GPU run (CUGAL_res.json: LOG, reg 0.3, <= 500
+    iterations, thr 1e-3, mu 2, 15 x 10 FW, sparse, SCIPY)."""
+    base = dict(sinkhorn_regularization=0.3, sinkhorn_threshold=1e-3, mu=2.0, iter_count=15,
+                frank_wolfe_iter_count=10, use_sparse_adjacency=True)
+    base.update(kw)
+    if device_cfg:
+        return Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                      sinkhorn_iterations=500, hungarian_method=HungarianMethod.SCIPY, **base)
+    return oracle.OracleConfig(sinkhorn_method="LOG", sinkhorn_iterations=500, hungarian_method="SCIPY", **base)
+
+
+def _oracle_run(source, target, cfg, threads=8):
+    # a fixed thread count: torch's CPU reductions split by thread count
     import copy
+
+    before = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        return oracle.align(copy.deepcopy(source), copy.deepcopy(target), cfg)
+    finally:
+        torch.set_num_threads(before)
+
+
+def _first_count_difference(device_profiles, run, n, reg, threshold, eval_freq=10):
+    """The first solve whose iteration count differs between the device and
+    the oracle, checked to be a threshold crossing within FP32 rounding: at
+    the check where the earlier side stopped, the other side's marginal
+    error is within the FP32 floor n 2^-24 (4 + max|C| / reg log2 e) of the
+    threshold (the FP32 potentials cannot resolve the error below it)."""
+    counts = [p.iteration_count for p in device_profiles]
+    for k, (a, b) in enumerate(zip(counts, run.sinkhorn_iterations)):
+        if a == b:
+            continue
+        floor = n * 2.0 ** -24 * (4.0 + run.cost_absmax[k] / reg * 1.4426950408889634)
+        stop = min(a, b)  # the earlier stop's iteration count: its check was at iteration stop - 1
+        index = (stop - 1) // eval_freq - 1  # the LOG checks are at iterations 10, 20, ...
+        other = run.sinkhorn_errors[k] if a < b else device_profiles[k].errors
+        err = other[index]
+        print(f"first differing solve {k}: device {a}, oracle {b} iterations; the other side's error at "
+              f"iteration {stop - 1} is {err:.3e} against threshold {threshold:g}, FP32 floor {floor:.2e}")
+        assert abs(err - threshold) <= floor, (k, err, floor)
+        return k
+    return None
+
+
+def test_reference_dataset_published_configuration_within_half_a_point(device):
+    # in-arenas (1133 nodes, 5400 edges, one self-loop; the reference's
+    # benchmark/data file, copied to tests/golden) with 10 % of source edges
+    # removed (seed 283841238) and the published configuration, end to end:
+    # node accuracy within 0.5 pp of the FP64 oracle, the same Sinkhorn
+    # iteration count for every one of the 150 solves.  (The FP32 CPU
+    # restatement agrees with FP64 here to 0.18 pp with equal counts: this
+    # instance is well conditioned.)
+    import copy
+    import os
+
+    from cugal_amd import cugal
+    from cugal_amd.profile import Profile
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    gen = np.random.default_rng(seed=283841238)
+    s, t = Graph(GraphKind.IN_ARENAS, {}).get(gen, golden)
+    source, target, mapping, _ = generate_graph(s, t, gen, NoiseLevel(0.1, 0.0, False))
+    profile = Profile()
+    _, answer = cugal(copy.deepcopy(source), copy.deepcopy(target), _published_config(True), profile)
+    run = _oracle_run(source, target, _published_config(False))
+    acc = float(np.mean(np.array([m for _, m in answer]) == np.asarray(mapping)))
+    ref_acc = float(np.mean(np.array([m for _, m in run.mapping]) == np.asarray(mapping)))
+    counts = [p.iteration_count for p in profile.sinkhorn_profiles]
+    print(f"\nin-arenas, published configuration: accuracy {acc:.4f} vs FP64 oracle {ref_acc:.4f}; Sinkhorn "
+          f"iterations {sum(counts)} vs {sum(run.sinkhorn_iterations)}")
+    first = _first_count_difference(profile.sinkhorn_profiles, run, 1133, 0.3, 1e-3)
+    assert first is None
+    assert abs(acc - ref_acc) <= 0.005 + 1e-12
+
+
+def test_harness_on_a_sensitive_reference_dataset(device):
+    # ca-netscience (379 nodes, 914 edges; the reference's benchmark/data
+    # file, copied to tests/golden) with the published configuration and
+    # the first noise level of the reference's run (10 % of source edges
+    # removed; seed 283841238), through the experiment harness.  A
+    # sensitivity check, held to 2 pp of node accuracy and of each structure
+    # score: on this instance the FP64 oracle itself moves with rounding
+    # (0.6201 on one host, 0.6016 on another with the same single thread;
+    # the FP32 CPU restatement of the same loop gives 0.6042), because many
+    # structurally equivalent nodes have nearly equal rows of Q.  Where the
+    # device and the oracle part (the first solve whose Sinkhorn iteration
+    # count differs), the stop is a threshold crossing within FP32 rounding:
+    # the FP32 floor of the marginal error (~0.009 here) is above the 1e-3
+    # threshold.  (The published 0.642 averages four instances from a build
+    # with the LOG marginal-check quirk; context, not a bound.)
     import os
 
     golden = os.path.join(os.path.dirname(__file__), "golden")
     seed = 283841238
     level = NoiseLevel(0.1, 0.0, False)
-    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
-                 sinkhorn_regularization=0.3, sinkhorn_iterations=500, sinkhorn_threshold=1e-3, mu=2.0,
-                 iter_count=15, frank_wolfe_iter_count=10, use_sparse_adjacency=True,
-                 hungarian_method=HungarianMethod.SCIPY)
-    exp = Experiment(algorithms=[Algorithm(cfg, False)], graphs=[Graph(GraphKind.CA_NETSCIENCE, {})],
-                     noise_levels=[level], seed=seed, num_runs=1, data_dir=golden)
+    exp = Experiment(algorithms=[Algorithm(_published_config(True), False)],
+                     graphs=[Graph(GraphKind.CA_NETSCIENCE, {})], noise_levels=[level], seed=seed, num_runs=1,
+                     data_dir=golden)
     result = exp.run().results[0][0][0]
 
     gen = np.random.default_rng(seed=seed)
     s, t = Graph(GraphKind.CA_NETSCIENCE, {}).get(gen, golden)
     assert (s.number_of_nodes(), s.number_of_edges()) == (379, 914)
     source, target, mapping, _ = generate_graph(s, t, gen, level)
-    ref = oracle.OracleConfig(sinkhorn_regularization=0.3, sinkhorn_method="LOG", sinkhorn_threshold=1e-3, mu=2.0,
-                              iter_count=15, frank_wolfe_iter_count=10, use_sparse_adjacency=True,
-                              hungarian_method="SCIPY")
-    run = oracle.align(copy.deepcopy(source), copy.deepcopy(target), ref)
+    run = _oracle_run(source, target, _published_config(False))
     answer = np.array([m for _, m in run.mapping])
     ref_acc = float(np.mean(answer == np.asarray(mapping)))
     ref_scores = _oracle_scores(source, target, answer, mapping)
     print(f"\nca-netscience, 10 % source noise: accuracy {result.accuracy:.4f} (device harness) vs "
           f"{ref_acc:.4f} (FP64 oracle); ICS/EC/S3 {result.ics:.4f}/{result.ec:.4f}/{result.sss:.4f} vs "
           f"{ref_scores[0]:.4f}/{ref_scores[1]:.4f}/{ref_scores[2]:.4f}")
+    _first_count_difference(result.profile.sinkhorn_profiles, run, 379, 0.3, 1e-3)
     assert abs(result.accuracy - ref_acc) <= 0.02
     for got, want in zip((result.ics, result.ec, result.sss), ref_scores):
         assert abs(got - want) <= 0.02
