@@ -42,6 +42,7 @@ _SIGNATURES = {
     "cg_node_features_workspace": (c_sz, [c_i64]),
     "cg_node_features": (c_int, [c_p, c_p, c_i64, c_f, c_p, c_p, c_p, c_sz, c_p]),
     "cg_egonet_edges": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "cg_edge_common_neighbours": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "cg_fugal_features_workspace": (c_sz, [c_i64]),
     "cg_fugal_features": (c_int, [c_p, c_p, c_i64, c_f, c_p, c_p, c_sz, c_p]),
     "cg_add_distance": (c_int, [c_p, c_p, c_i64, c_p, c_p]),

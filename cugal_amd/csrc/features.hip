@@ -141,6 +141,22 @@ __global__ __launch_bounds__(256) void vertex_triangles(const int32_t* __restric
     }
 }
 
+// Common neighbours of every CSR entry (v, u): |N(v) & N(u)|, one wave per
+// vertex, lanes over its neighbours.
+__global__ __launch_bounds__(256) void entry_common_neighbours(const int32_t* __restrict__ col,
+                                                              const int32_t* __restrict__ rp, int n,
+                                                              int32_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * 4 + threadIdx.x / 64;
+    if (v >= n) return;
+    const int b = rp[v];
+    const int e = rp[v + 1];
+    for (int k = b + lane; k < e; k += 64) {
+        const int u = col[k];
+        out[k] = intersect_count(col + b, e - b, col + rp[u], rp[u + 1] - rp[u]);
+    }
+}
+
 __device__ __forceinline__ double clustering_of(const int32_t* __restrict__ rp, const int32_t* __restrict__ tri,
                                                 int u) {
     const int d = rp[u + 1] - rp[u];
@@ -431,6 +447,15 @@ int cg_fugal_features(const int32_t* col_indices, const int32_t* row_pointers, i
     hipLaunchKernelGGL(fugal_extra, dim3(blocks), dim3(kFugalThreads), lds, s, col_indices, row_pointers, tri, f4, ni,
                        mu, out);
     return check_launch("fugal_extra");
+}
+
+int cg_edge_common_neighbours(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, int32_t* out,
+                              cg_stream_t stream) {
+    if (!row_pointers || !out || n <= 0 || n > (int64_t(1) << 30))
+        return fail(CG_EINVAL, "cg_edge_common_neighbours: bad arguments");
+    hipLaunchKernelGGL(entry_common_neighbours, dim3(static_cast<unsigned>(div_up(n, 4))), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), col_indices, row_pointers, static_cast<int>(n), out);
+    return check_launch("entry_common_neighbours");
 }
 
 }  // extern "C"

@@ -16,9 +16,15 @@ and its quasi-permutation is copied back to the callers' labels
 (cg_permute_copy) before the LAP, which sees the original labels.
 
 The order is reverse Cuthill-McKee (breadth-first levels, neighbours of
-earlier nodes first), computed on the host from the CSR; "auto" keeps it for
-a graph only where it puts clearly more edges within a window of labels than
-the graph's own labels do.
+earlier nodes first), computed on the host from the CSR, either of the whole
+graph or of its edges that close at least two triangles: the shortcuts of a
+small-world graph close none, and numbering without them follows the
+graph's local structure (a rewired ring lattice gets its ring order back,
+89 % of the edges within 64 labels, against 27 % for the plain order).
+"auto" keeps the candidate (own labels, RCM, filtered RCM) that puts the
+most edges within a small window (64 labels: an LDS tile's reach) plus a
+large one (min(n/4, 8192) labels: an L2's reach of a gathered strip), and
+relabels only where that gains clearly over the graph's own labels.
 """
 
 from __future__ import annotations
@@ -48,9 +54,16 @@ def _host_csr(adj: Adjacency):
     return rp, col
 
 
-def window_locality(rp: np.ndarray, col: np.ndarray, position: np.ndarray | None = None) -> float:
-    """Fraction of CSR entries (r, c) with |pos(r) - pos(c)| <= min(n/4,
-    8192); `position` maps each node to its label (identity when None)."""
+SMALL_WINDOW = 64
+# edges closing fewer triangles than this are left out of the filtered order
+FILTER_MIN_COMMON = 2
+
+
+def window_locality(rp: np.ndarray, col: np.ndarray, position: np.ndarray | None = None,
+                    window: int | None = None) -> float:
+    """Fraction of CSR entries (r, c) with |pos(r) - pos(c)| <= window
+    (default min(n/4, 8192)); `position` maps each node to its label
+    (identity when None)."""
     n = len(rp) - 1
     if len(col) == 0:
         return 1.0
@@ -59,8 +72,14 @@ def window_locality(rp: np.ndarray, col: np.ndarray, position: np.ndarray | None
         rows, cols = position[rows], position[col]
     else:
         cols = col
-    window = min(max(n // 4, 1), 8192)
+    if window is None:
+        window = min(max(n // 4, 1), 8192)
     return float(np.mean(np.abs(rows - cols) <= window))
+
+
+def locality_score(rp: np.ndarray, col: np.ndarray, position: np.ndarray | None = None) -> float:
+    """Small-window plus large-window locality (0 to 2)."""
+    return window_locality(rp, col, position, SMALL_WINDOW) + window_locality(rp, col, position)
 
 
 def rcm_position(rp: np.ndarray, col: np.ndarray) -> np.ndarray:
@@ -73,19 +92,44 @@ def rcm_position(rp: np.ndarray, col: np.ndarray) -> np.ndarray:
     return position
 
 
+def filtered_rcm_position(adj: Adjacency) -> np.ndarray:
+    """Reverse Cuthill-McKee of the graph without its edges that have fewer
+    than FILTER_MIN_COMMON common neighbours (counted on the device,
+    cg_edge_common_neighbours); the edge filter is symmetric, so the
+    filtered graph is undirected too."""
+    n = adj.number_of_nodes()
+    dev = adj.row_pointers.device
+    nnz = adj.col_indices.numel()
+    common = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    nat.call("cg_edge_common_neighbours", nat.ptr(adj.col_indices), nat.ptr(adj.row_pointers), n, nat.ptr(common),
+             nat.stream_of(dev))
+    keep = (common >= FILTER_MIN_COMMON).cpu().numpy()
+    rp, col = _host_csr(adj)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    M = scipy.sparse.csr_matrix((np.ones(int(keep.sum()), dtype=np.int8), (rows[keep], col[keep])), shape=(n, n))
+    order = scipy.sparse.csgraph.reverse_cuthill_mckee(M, symmetric_mode=True)
+    position = np.empty(n, dtype=np.int64)
+    position[order] = np.arange(n, dtype=np.int64)
+    return position
+
+
 def choose_position(adj: Adjacency, mode: str) -> np.ndarray | None:
     """The relabelling of `adj` for the gathers (new label of each node), or
-    None to keep its labels."""
+    None to keep its labels: the candidate (own labels, RCM, RCM of the
+    triangle-closing edges) with the best locality score; "rcm" takes the
+    better RCM even when the own labels score higher."""
     if mode not in ORDERS:
         raise ValueError(f"node_order must be one of {ORDERS}, got {mode!r}")
     n = adj.number_of_nodes()
     if mode == "natural" or n < 2 or adj.directed or (mode == "auto" and n < AUTO_MIN_NODES):
         return None
     rp, col = _host_csr(adj)
-    position = rcm_position(rp, col)
-    if mode == "auto" and window_locality(rp, col, position) < window_locality(rp, col) + AUTO_MIN_GAIN:
+    candidates = [rcm_position(rp, col), filtered_rcm_position(adj)]
+    scores = [locality_score(rp, col, p) for p in candidates]
+    best = int(np.argmax(scores))
+    if mode == "auto" and scores[best] < locality_score(rp, col) + AUTO_MIN_GAIN:
         return None
-    return position
+    return candidates[best]
 
 
 def relabel(adj: Adjacency, position: torch.Tensor) -> Adjacency:

@@ -94,6 +94,14 @@ size_t cg_node_features_workspace(int64_t n);
 int cg_node_features(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, float mu, float* out,
                      int32_t* triangles, void* workspace, size_t workspace_bytes, cg_stream_t stream);
 
+/* Per CSR entry (v, u) the number of common neighbours |N(v) & N(u)| (out:
+ * int32, one per entry of col_indices; rows sorted).  No reference
+ * counterpart: the locality order of the gradient's gathers drops the edges
+ * that close no triangle (the shortcuts of a small-world graph) before its
+ * breadth-first numbering. */
+int cg_edge_common_neighbours(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, int32_t* out,
+                              cg_stream_t stream);
+
 /* Ego-net edge count per vertex, deg(v) + t(v) / 2: the feature of
  * FUGAL/pred.py:68-72 (nx.ego_graph(G, v).number_of_edges() on a simple
  * graph).  Needs `triangles` from cg_graph_features / cg_node_features. */

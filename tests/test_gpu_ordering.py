@@ -63,3 +63,35 @@ def test_gradient_is_equivariant_under_relabelling(device):
     ref.sort_indices()
     assert np.array_equal(B2.row_pointers.cpu().numpy(), ref.indptr)
     assert np.array_equal(B2.col_indices.cpu().numpy(), ref.indices)
+
+
+def test_locality_order_choice(device):
+    # A Watts-Strogatz graph (k = 20, 10 % rewired) under a random
+    # relabelling: plain RCM keeps few edges within 64 labels (the
+    # shortcuts spread its breadth-first levels); RCM of the edges that
+    # close at least two triangles gets the ring order back.  "auto" picks
+    # it for the shuffled copy and keeps the ring's own labels.
+    n = 32768
+    ring = nx.watts_strogatz_graph(n, 20, 0.1, seed=0)
+    shuffled = nx.relabel_nodes(ring, dict(zip(range(n), np.random.default_rng(1).permutation(n).tolist())))
+    a, b = Adjacency.from_graph(ring, device), Adjacency.from_graph(shuffled, device)
+    rp, col = ordering._host_csr(b)
+    plain = ordering.rcm_position(rp, col)
+    filtered = ordering.filtered_rcm_position(b)
+    assert np.array_equal(np.sort(filtered), np.arange(n))
+    small = [ordering.window_locality(rp, col, p, 64) for p in (None, plain, filtered)]
+    print(f"edges within 64 labels: own {small[0]:.3f}, RCM {small[1]:.3f}, filtered RCM {small[2]:.3f}")
+    assert small[0] < 0.05 and small[1] < 0.5 < 0.85 < small[2]
+    assert np.array_equal(ordering.choose_position(b, "auto"), filtered)
+    assert ordering.choose_position(a, "auto") is None
+    # the common-neighbour counts against scipy, (M M) * M entrywise
+    M = nx.to_scipy_sparse_array(shuffled, nodelist=range(n), format="csr")
+    C = (M @ M).multiply(M).tocoo()
+    lookup = {(int(r), int(c)): int(v) for r, c, v in zip(C.row, C.col, C.data)}
+    common = torch.empty(b.col_indices.numel(), dtype=torch.int32, device=device)
+    nat.call("cg_edge_common_neighbours", nat.ptr(b.col_indices), nat.ptr(b.row_pointers), n, nat.ptr(common),
+             nat.stream_of(device))
+    common = common.cpu().numpy()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    want = np.array([lookup.get((int(r), int(c)), 0) for r, c in zip(rows, col)])
+    assert np.array_equal(common, want)
