@@ -297,7 +297,7 @@ int cg_sinkhorn_unrolled_backward(const float* C, int64_t n, float regularizatio
  * cugal/pred.py:45-46) this runs on MFMA; each product carries a relative
  * error of about 2^-16 from the hi+lo split.  a_bits, b_bits:
  * cg_adjacency_bits_words(n) words each, from cg_adjacency_pack_bits: the
- * adjacency as bits, zero padded to np = n rounded up to 256; word
+ * adjacency as bits, zero padded to np = n rounded up to 512; word
  * (k / 32) * np + i holds bits k % 32 of row i.  Undirected graphs only. */
 size_t cg_adjacency_bits_words(int64_t n);
 int cg_adjacency_pack_bits(const int32_t* col_indices, const int32_t* row_pointers, int64_t n, uint32_t* out,
