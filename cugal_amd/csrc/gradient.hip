@@ -15,6 +15,9 @@
 //      (B symmetric, so (B Y)^T = (A P) B).
 // Tiles are dealt so that the blocks one XCD runs together cover
 // consecutive row tiles of one 32-column strip, which fits that XCD's L2.
+// For undirected pairs both products run gather_win_t, which serves the
+// neighbours near a tile (most of them, on locally ordered labels) from a
+// window of rows staged in LDS.
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -165,6 +168,65 @@ struct SecondList {
     const float* X;
 };
 
+// Writes the gathered 64 x TC tile (tile[r][c] = Z[r0 + r][c0 + c])
+// transposed: out[c][r] = Z[r][c], or with EPI the gradient terms
+// scale * Z + D(c, r) + rl (1 - 2 P[c][r]).  Callers synchronise first.
+template <bool EPI, bool VEC, int FD, int TC>
+__device__ __forceinline__ void store_tile_t(const float (*tile)[TC + 1], int r0, int c0, int n, float scale,
+                                             const Epilogue& ep, float* __restrict__ out, int64_t ldo) {
+    const int g = threadIdx.x >> 4;  // 16 output rows per pass
+    const int q = threadIdx.x & 15;  // four consecutive output columns
+    const int rq = r0 + 4 * q;
+    // A thread's four output columns are fixed: their target feature rows
+    // are loaded once, not once per output row.
+    FeatureRow ftj[4];
+    if (EPI && ep.fs) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ftj[e] = rq + e < n ? feature_row<FD>(ep.ft, rq + e) : FeatureRow{};
+    }
+    for (int cc = g; cc < TC; cc += 16) {
+        const int c = c0 + cc;
+        if (c >= n) break;
+        float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
+        const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
+        const int64_t base = static_cast<int64_t>(c) * ldo;
+        if (EPI) {
+            // Same terms in the same order as epi_one: scale * sum, + D,
+            // + rl (1 - 2 P); P read as one 16-byte load where aligned.
+            FeatureRow fsi{};
+            if (ep.fs) fsi = feature_row<FD>(ep.fs, c);
+            float pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (ep.P) {
+                if (VEC && ep.pvec && rq + 3 < n) {
+                    const float4 p4 = *reinterpret_cast<const float4*>(ep.P + pbase + rq);
+                    pv[0] = p4.x;
+                    pv[1] = p4.y;
+                    pv[2] = p4.z;
+                    pv[3] = p4.w;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (rq + e < n) pv[e] = ep.P[pbase + rq + e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float gv = v[e] * scale;
+                if (ep.fs) gv += feature_distance_rows<FD>(fsi, ftj[e]);
+                if (ep.P) gv += ep.rl - 2.0f * ep.rl * pv[e];
+                v[e] = gv;
+            }
+        }
+        if (VEC && rq + 3 < n) {
+            st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (rq + e < n) out[base + rq + e] = v[e];
+        }
+    }
+}
+
 template <bool EPI, bool VEC, int FD, int TC>
 __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __restrict__ rp,
                                                            const int32_t* __restrict__ col, int n,
@@ -248,57 +310,235 @@ __global__ __launch_bounds__(kSpThreads) void gather_tile_t(const int32_t* __res
         }
     }
     __syncthreads();
-    const int g = threadIdx.x >> 4;  // 16 output rows per pass
-    const int q = threadIdx.x & 15;  // four consecutive output columns
-    const int rq = r0 + 4 * q;
-    // A thread's four output columns are fixed: their target feature rows
-    // are loaded once, not once per output row.
-    FeatureRow ftj[4];
-    if (EPI && ep.fs) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) ftj[e] = rq + e < n ? feature_row<FD>(ep.ft, rq + e) : FeatureRow{};
+    store_tile_t<EPI, VEC, FD, TC>(tile, r0, c0, n, scale, ep, out, ldo);
+}
+
+// ---- tiled gather through an LDS window ----------------------------------
+// Where the labels are local (reverse Cuthill-McKee order, a ring lattice's
+// own order), most neighbours of a 64-row tile lie within kHalo rows of it.
+// gather_win_t stages the tile's window of X, rows [r0 - kHalo, r0 + 64 +
+// kHalo) x 32 columns (12 KiB), in LDS with buffer loads, and gathers those
+// neighbours from LDS; only the others ("far") are loaded from global
+// memory.  The texture addresser, which bounds the plain gather (one
+// 16-byte lane load per neighbour and column quad: 79 % busy at n = 32768),
+// then handles the window's 96 row segments plus the far ones instead of
+// every neighbour's.  The small window keeps 7 blocks per CU, which hides
+// the tile's dependent load phases (a 192 x 64 window, 3 blocks per CU, was
+// twice as slow as the plain gather).  A row's local entries are one run
+// [lb, le) of its sorted column list, found once per call by window_split.
+constexpr int kHalo = 16;
+constexpr int kWinR = kTileR + 2 * kHalo;  // 96 window rows
+constexpr int kWinTC = 32;                 // columns per tile: 8 lanes x 16 bytes per row
+constexpr int kWinIdxChunk = 2048;         // staged column indices: a whole tile's lists at degree 32
+constexpr int kFarBatch = 2;               // far loads per row and round
+constexpr int kRsrcWord3 = 0x00020000;     // raw buffer descriptor word 3 on CDNA
+
+// split[r] = (lb, le): the entries [lb, le) of row r whose columns lie in the
+// window of its row tile, [64 (r / 64) - kHalo, 64 (r / 64) + 64 + kHalo).
+// Binary searches assume a sorted row; the run is then checked, and a row
+// whose run holds a column outside the window (an unsorted row) gets an
+// empty run, so all its entries are gathered from global memory.
+__global__ __launch_bounds__(kSpThreads) void window_split(const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ col, int n, int halo,
+                                                          int2* __restrict__ split) {
+    const int r = static_cast<int>(blockIdx.x) * kSpThreads + static_cast<int>(threadIdx.x);
+    if (r >= n) return;
+    const int wlo = (r / kTileR) * kTileR - halo;
+    const int whi = wlo + kTileR + 2 * halo;
+    const int b = rp[r], e = rp[r + 1];
+    int lo = b, hi = e;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < wlo) lo = mid + 1; else hi = mid;
     }
-    for (int cc = g; cc < TC; cc += 16) {
-        const int c = c0 + cc;
-        if (c >= n) break;
-        float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
-        const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
-        const int64_t base = static_cast<int64_t>(c) * ldo;
-        if (EPI) {
-            // Same terms in the same order as epi_one: scale * sum, + D,
-            // + rl (1 - 2 P); P read as one 16-byte load where aligned.
-            FeatureRow fsi{};
-            if (ep.fs) fsi = feature_row<FD>(ep.fs, c);
-            float pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (ep.P) {
-                if (VEC && ep.pvec && rq + 3 < n) {
-                    const float4 p4 = *reinterpret_cast<const float4*>(ep.P + pbase + rq);
-                    pv[0] = p4.x;
-                    pv[1] = p4.y;
-                    pv[2] = p4.z;
-                    pv[3] = p4.w;
-                } else {
+    const int lb = lo;
+    hi = e;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < whi) lo = mid + 1; else hi = mid;
+    }
+    int le = lo;
+    for (int k = lb; k < le; ++k) {
+        const int c = col[k];
+        if (c < wlo || c >= whi) {
+            le = lb;
+            break;
+        }
+    }
+    split[r] = make_int2(lb, le);
+}
+
+template <bool EPI, int FD>
+__global__ __launch_bounds__(kSpThreads) void gather_win_t(const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ col,
+                                                          const int2* __restrict__ split, int n,
+                                                          const float* __restrict__ X, int64_t ldx, float scale,
+                                                          Epilogue ep, float* __restrict__ out, int64_t ldo,
+                                                          int tiles_r, int64_t tiles) {
+    constexpr int TC = kWinTC;
+    constexpr int WINR = kWinR;
+    constexpr int HALO = kHalo;
+    constexpr int CHUNK = kWinIdxChunk;
+    constexpr int FB = kFarBatch;
+    constexpr int LPT = TC / 4;           // 8 lanes per row segment
+    constexpr int RG = kSpThreads / LPT;  // 32 row groups
+    constexpr int RPT = kTileR / RG;      // 2 rows per thread
+    // The window, later reused for the transposed tile (64 x 33 floats).
+    __shared__ __attribute__((aligned(16))) float win[(WINR * TC > kTileR * (TC + 1)) ? WINR * TC : kTileR * (TC + 1)];
+    __shared__ int sidx[CHUNK];
+    __shared__ int srp[kTileR + 1];
+    __shared__ int slb[kTileR], sle[kTileR];
+    __shared__ int s_local;
+    const int64_t t = xcd_tile(blockIdx.x, tiles);
+    if (t >= tiles) return;
+    const int r0 = static_cast<int>(t % tiles_r) * kTileR;
+    const int c0 = static_cast<int>(t / tiles_r) * TC;
+    const int wbase = r0 - HALO;
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid <= kTileR) srp[tid] = rp[min(r0 + tid, n)];
+    if (tid < kTileR) {
+        const int r = r0 + tid;
+        const int2 sp = r < n ? split[r] : make_int2(rp[n], rp[n]);
+        slb[tid] = sp.x;
+        sle[tid] = sp.y;
+        const int local = wave_sum(sp.y - sp.x);  // wave 0 holds the tile's rows
+        if (tid == 0) s_local = local;
+    }
+    __syncthreads();
+    // Staging costs the window's kWinR row segments; it pays when the tile
+    // has more local entries than that.
+    const bool use_win = s_local > WINR;
+    if (use_win) {
+        // Window row i is X row clamp(wbase + i, 0, n - 1); rows outside
+        // [0, n) are never listed, so their contents do not matter.  Load ls
+        // of wave w fills window rows RPW (w + 4 ls) .. + RPW - 1 (1 KiB);
+        // the descriptor's range ends at the last window row's column n, so
+        // column quads past n read zeros.
+        const int base_row = max(wbase, 0);
+        const int last_row = min(wbase + WINR - 1, n - 1);
+        const uint32_t nrec = static_cast<uint32_t>(((last_row - base_row) * ldx + n - c0) * 4);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(X + static_cast<int64_t>(base_row) * ldx + c0), 0, nrec, kRsrcWord3);
+        const int wave = tid >> 6, lane = tid & 63;
+        constexpr int RPW = 1024 / (TC * 4);  // window rows per 1 KiB wave load
+        static_assert(WINR % (4 * RPW) == 0, "the window must split into whole wave loads");
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (rq + e < n) pv[e] = ep.P[pbase + rq + e];
+        for (int ls = 0; ls < WINR / (4 * RPW); ++ls) {
+            const int wr = RPW * (wave + 4 * ls) + lane / LPT;
+            const int grow = min(max(wbase + wr, 0), n - 1);
+            const uint32_t voff = static_cast<uint32_t>((static_cast<int64_t>(grow - base_row) * ldx + 4 * (lane % LPT)) * 4);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(win) + 1024 * (wave + 4 * ls)),
+                16, voff, 0, 0, 0);
+        }
+    }
+    const int g = tid / LPT;  // row group
+    const int q = tid % LPT;  // float4 slot within the segment
+    const int cq = c0 + 4 * q;
+    float4 acc[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) acc[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int e_lo = srp[0];
+    const int e_hi = srp[kTileR];
+    bool window_ready = !use_win;
+    for (int cs = e_lo; cs < e_hi; cs += CHUNK) {
+        const int ce = min(e_hi, cs + CHUNK);
+        for (int k = cs + tid; k < ce; k += kSpThreads) sidx[k - cs] = col[k];
+        __syncthreads();
+        // Far entries: [row start, lb) and [le, row end) (the whole row
+        // when the window is not used), FB per row and side in flight
+        // across the thread's rows before any is added (measured faster
+        // than eight in flight per row, also on graphs with no window).
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int kb[RPT], ke[RPT];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int rr = g + r * RG;
+                const int a = side == 0 ? srp[rr] : (use_win ? sle[rr] : srp[rr + 1]);
+                const int b = side == 0 ? (use_win ? slb[rr] : srp[rr + 1]) : srp[rr + 1];
+                kb[r] = max(a, cs);
+                ke[r] = min(b, ce);
+            }
+            for (int off = 0;; off += FB) {
+                bool any = false;
+                float4 x[RPT][FB];
+#pragma unroll
+                for (int r = 0; r < RPT; ++r)
+#pragma unroll
+                    for (int u = 0; u < FB; ++u) {
+                        const int k = kb[r] + off + u;
+                        x[r][u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (k < ke[r]) {
+                            any = true;
+                            if (cq < n) x[r][u] = *reinterpret_cast<const float4*>(X + sidx[k - cs] * ldx + cq);
+                        }
+                    }
+                if (!any) break;
+#pragma unroll
+                for (int r = 0; r < RPT; ++r)
+#pragma unroll
+                    for (int u = 0; u < FB; ++u)
+                        if (kb[r] + off + u < ke[r]) {
+                            acc[r].x += x[r][u].x;
+                            acc[r].y += x[r][u].y;
+                            acc[r].z += x[r][u].z;
+                            acc[r].w += x[r][u].w;
+                        }
+            }
+        }
+        if (!window_ready) {
+            // The window's buffer loads have landed in every wave.
+            __builtin_amdgcn_s_waitcnt(0x0070);
+            __syncthreads();
+            window_ready = true;
+        }
+        if (use_win) {
+            const float4* __restrict__ w4 = reinterpret_cast<const float4*>(win);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int rr = g + r * RG;
+                const int k0 = max(slb[rr], cs), k1 = min(sle[rr], ce);
+                int k = k0;
+                for (; k + 8 <= k1; k += 8) {
+                    float4 x[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) x[u] = w4[(sidx[k - cs + u] - wbase) * LPT + q];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        acc[r].x += x[u].x;
+                        acc[r].y += x[u].y;
+                        acc[r].z += x[u].z;
+                        acc[r].w += x[u].w;
+                    }
+                }
+                for (; k < k1; ++k) {
+                    const float4 x = w4[(sidx[k - cs] - wbase) * LPT + q];
+                    acc[r].x += x.x;
+                    acc[r].y += x.y;
+                    acc[r].z += x.z;
+                    acc[r].w += x.w;
                 }
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float gv = v[e] * scale;
-                if (ep.fs) gv += feature_distance_rows<FD>(fsi, ftj[e]);
-                if (ep.P) gv += ep.rl - 2.0f * ep.rl * pv[e];
-                v[e] = gv;
-            }
         }
-        if (VEC && rq + 3 < n) {
-            st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (rq + e < n) out[base + rq + e] = v[e];
-        }
+        __syncthreads();
     }
+    if (!window_ready) {
+        // A tile without entries still drains its window loads before the
+        // window's memory is reused.
+        __builtin_amdgcn_s_waitcnt(0x0070);
+        __syncthreads();
+    }
+    float (*tile)[TC + 1] = reinterpret_cast<float (*)[TC + 1]>(win);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        tile[g + r * RG][4 * q + 0] = acc[r].x;
+        tile[g + r * RG][4 * q + 1] = acc[r].y;
+        tile[g + r * RG][4 * q + 2] = acc[r].z;
+        tile[g + r * RG][4 * q + 3] = acc[r].w;
+    }
+    __syncthreads();
+    store_tile_t<EPI, true, FD, TC>(tile, r0, c0, n, scale, ep, out, ldo);
 }
 
 // out (row stride ldo) = X (row stride ldx): row-major streaming copy.
@@ -427,6 +667,34 @@ int launch_gather_tile(const int32_t* rp, const int32_t* col, int n, const float
     return launch_gather_tile_fd<EPI, 4, 32>(rp, col, n, X, ldx, scale, ep, out, ldo, s, second);
 }
 
+// The window gather where its 16-byte accesses are aligned (else returns 1
+// without launching, and the caller takes the plain gather).
+// The window gather where its 16-byte accesses are aligned (else returns 0
+// without launching, and the caller takes the plain gather).
+template <bool EPI>
+int launch_gather_win(const int32_t* rp, const int32_t* col, const int2* split, int n, const float* X, int64_t ldx,
+                      float scale, const Epilogue& ep, float* out, int64_t ldo, hipStream_t s, bool& launched) {
+    launched = false;
+    const bool vec = (n % 4 == 0) && (ldx % 4 == 0) && (ldo % 4 == 0) && aligned16(X) && aligned16(out) &&
+                     (!ep.P || aligned16(ep.P)) && (!ep.fs || (aligned16(ep.fs) && aligned16(ep.ft)));
+    if (!vec) return 0;
+    const int tiles_r = static_cast<int>(div_up(n, kTileR));
+    const int64_t tiles = static_cast<int64_t>(tiles_r) * div_up(n, kWinTC);
+    const int64_t blocks = div_up(tiles, kNumXcd) * kNumXcd;
+    if (blocks > 0x7fffffffLL) return fail(CG_EUNSUPPORTED, "gradient grid too large (n=%d)", n);
+    launched = true;
+    if constexpr (EPI) {
+        if (ep.fs && ep.fdim == 8) {
+            hipLaunchKernelGGL((gather_win_t<EPI, 8>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp,
+                               col, split, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
+            return check_launch("gather_win_t");
+        }
+    }
+    hipLaunchKernelGGL((gather_win_t<EPI, 4>), dim3(static_cast<unsigned>(blocks)), dim3(kSpThreads), 0, s, rp, col,
+                       split, n, X, ldx, scale, ep, out, ldo, tiles_r, tiles);
+    return check_launch("gather_win_t");
+}
+
 int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float* X, float scale, const Epilogue& ep,
                      float* out, hipStream_t s) {
     const bool vec = (n % 4 == 0) && aligned16(X) && aligned16(out) && (!ep.P || aligned16(ep.P)) &&
@@ -518,10 +786,12 @@ static int64_t gradient_ld(int64_t n) {
 // A row stride whose 128-byte column segments spread over the L2 sets.
 static bool gather_friendly_ld(int64_t ld) { return ld % 4 == 0 && (ld * 4) % 4096 != 0; }
 
-// Two padded n x n buffers: a copy of P (gathered by the first product) and
-// the intermediate Y.
+// Two padded n x n buffers, a copy of P (gathered by the first product) and
+// the intermediate Y, and the window runs of both adjacencies.
 size_t cg_gradient_workspace(int64_t n) {
-    return n > 0 ? 2 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) : 0;
+    return n > 0 ? 2 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) +
+                       2 * (sizeof(int2) * static_cast<size_t>(n) + 256)
+                 : 0;
 }
 
 int64_t cg_gradient_padded_ld(int64_t n) { return n > 0 ? gradient_ld(n) : 0; }
@@ -562,14 +832,35 @@ int gradient_impl(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* 
         X1 = Pp;
         ld1 = ldy;
     }
-    // Y = (A P)^T; directed also Y2 = (A^T P)^T
+    if (!directed) {
+        // Symmetric pairs: both products through the LDS window where the
+        // tiles' neighbours are local enough (each tile decides).
+        int2* a_split = cv.take<int2>(static_cast<size_t>(n));
+        int2* b_split = cv.take<int2>(static_cast<size_t>(n));
+        const unsigned sb = static_cast<unsigned>(div_up(n, kSpThreads));
+        hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, a_rowptr, a_col, ni, kHalo, a_split);
+        if (int rc = check_launch("window_split")) return rc;
+        hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, b_rowptr, b_col, ni, kHalo, b_split);
+        if (int rc = check_launch("window_split")) return rc;
+        bool launched = false;
+        if (int rc = launch_gather_win<false>(a_rowptr, a_col, a_split, ni, X1, ld1, 1.0f, none, Y, ldy, s, launched))
+            return rc;
+        if (!launched)
+            if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
+        const float rl = regularizer_factor * static_cast<float>(lambda_);
+        Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim,
+                    aligned16(P) && ld_p % 4 == 0 ? 1 : 0};
+        if (int rc = launch_gather_win<true>(b_rowptr, b_col, b_split, ni, Y, ldy, -2.0f, ep, gradient, n, s, launched))
+            return rc;
+        if (launched) return 0;
+        return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
+    }
+    // Directed: Y = (A P)^T, Y2 = (A^T P)^T
     if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
-    if (directed)
-        if (int rc = launch_gather_tile<false>(at_rowptr, at_col, ni, X1, ld1, 1.0f, none, Y2, ldy, s)) return rc;
+    if (int rc = launch_gather_tile<false>(at_rowptr, at_col, ni, X1, ld1, 1.0f, none, Y2, ldy, s)) return rc;
     const float rl = regularizer_factor * static_cast<float>(lambda_);
     Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim,
                 aligned16(P) && ld_p % 4 == 0 ? 1 : 0};
-    if (!directed) return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
     // (B Y + B^T Y2)^T = A P B^T + A^T P B, summed in one tile pass
     return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -1.0f, ep, gradient, n, s,
                                     SecondList{bt_rowptr, bt_col, Y2});
