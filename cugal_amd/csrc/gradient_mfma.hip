@@ -68,8 +68,11 @@ constexpr int kSplitGroup = 2;  // split panels per group in the block order
 // format otherwise unused by raw loads).
 constexpr int kRsrcWord3 = 0x00020000;
 // s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
-// 11:8): all but the newest stage's 9 loads and no LDS reads pending; all.
-constexpr unsigned kWaitNewestStage = 0x0079;
+// 11:8): all but the newest kStages - 2 stages' loads and no LDS
+// reads pending; all.
+constexpr unsigned kLoadsPerStage = 9;  // per thread: 4 hi + 4 lo + 1 words
+constexpr unsigned kLoadsInFlight = kLoadsPerStage * (kStages - 2);
+constexpr unsigned kWaitNewestStage = (kLoadsInFlight & 15u) | ((kLoadsInFlight >> 4) << 14) | 0x0070u;
 constexpr unsigned kWaitAll = 0x0070;
 constexpr int kNumXcd = 8;
 
@@ -226,7 +229,6 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
             rw, (lds_ptr)(lds + stage * kStageB + 2 * kSplitB + plane * 2048 + (wave & 1) * 1024), 16, wvoff, 0, 0, 0);
     };
-    constexpr int kLoadsPerStage = 9;
     // MFMAs of a stage's last k step issued after the barrier.
     constexpr int kTailMfma = 8;
 
@@ -307,14 +309,14 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
         }
     };
 
-    // Pipeline: stage kt % 3 holds k slice kt (64 wide).  Iteration kt
-    // starts the loads of slice kt + 2, runs its four k steps, then waits
-    // for slice kt + 1 (all but the newest kLoadsPerStage loads) and
-    // synchronises.  The last iterations re-fetch the final slice into a
+    // Pipeline: stage kt % kStages holds k slice kt (64 wide).  Iteration
+    // kt starts the loads of slice kt + kStages - 1 into the stage read in
+    // iteration kt - 1, runs its four k steps, then waits for slice kt + 1
+    // (all but the newest kStages - 2 stages' loads) and synchronises.  The last iterations re-fetch the final slice into a
     // stage no longer read (in bounds), so the load count per iteration, and
     // the wait, stay fixed.
-    issue_loads(0, 0);
-    issue_loads(min(1, nst - 1), 1);
+#pragma unroll
+    for (int p = 0; p < kStages - 1; ++p) issue_loads(min(p, nst - 1), p);
     __builtin_amdgcn_s_waitcnt(kWaitNewestStage);
     __syncthreads();
     uint32_t words[2][4];
@@ -329,8 +331,8 @@ __global__ __launch_bounds__(kThreads) void gemm_nt(const uint32_t* __restrict__
     int stage = 0;
     for (int kt = 0; kt < nst; ++kt) {
         const int stage1 = stage == kStages - 1 ? 0 : stage + 1;
-        const int stage2 = stage1 == kStages - 1 ? 0 : stage1 + 1;
-        issue_loads(min(kt + 2, nst - 1), stage2);
+        const int refill = stage == 0 ? kStages - 1 : stage - 1;  // read in iteration kt - 1
+        issue_loads(min(kt + kStages - 1, nst - 1), refill);
         const char* base = lds + stage * kStageB;
         step_with_prefetch(base, words, 0, ax, ah, al, bx, bh, bl, 8);
         step_with_prefetch(base, words, 1, bx, bh, bl, ax, ah, al, 8);
