@@ -18,6 +18,8 @@
 // For undirected pairs both products run gather_win_t, which serves the
 // neighbours near a tile (most of them, on locally ordered labels) from a
 // window of rows staged in LDS.
+#include <mutex>
+
 #include "common.hpp"
 #include "cugal_hip.h"
 
@@ -330,7 +332,8 @@ constexpr int kHalo = 16;
 constexpr int kWinR = kTileR + 2 * kHalo;  // 96 window rows
 constexpr int kWinTC = 32;                 // columns per tile: 8 lanes x 16 bytes per row
 constexpr int kWinIdxChunk = 2048;         // staged column indices: a whole tile's lists at degree 32
-constexpr int kFarBatch = 2;               // far loads per row and round
+constexpr int kFarWindow = 4;              // far loads in flight per thread
+constexpr int kLongRow = 64;               // a graph with a longer list (a hub) takes gather_tile_t
 constexpr int kRsrcWord3 = 0x00020000;     // raw buffer descriptor word 3 on CDNA
 
 // split[r] = (lb, le): the entries [lb, le) of row r whose columns lie in the
@@ -368,6 +371,24 @@ __global__ __launch_bounds__(kSpThreads) void window_split(const int32_t* __rest
     split[r] = make_int2(lb, le);
 }
 
+// *hub = 1 when some row lists more than kLongRow entries (one block).  The
+// window kernel walks a thread's lists kFarWindow entries per round, which
+// suits short lists; a hub's long list would hold its tile for hundreds of
+// rounds (Barabasi-Albert n = 8192: 1.25 ms per gradient against 0.78 ms
+// with gather_tile_t, whose rows keep eight loads in flight), so a graph
+// with a hub is gathered by gather_tile_t.  The choice is made per graph
+// on the host: both a hub branch inside the window kernel and a
+// tile-walking form of both kernels, either of which would avoid the host
+// read, measured slower.
+constexpr int kHubThreads = 1024;
+__global__ __launch_bounds__(kHubThreads) void graph_has_hub(const int32_t* __restrict__ rp, int n,
+                                                            int32_t* __restrict__ hub) {
+    int found = 0;
+    for (int r = static_cast<int>(threadIdx.x); r < n; r += kHubThreads) found |= (rp[r + 1] - rp[r]) > kLongRow;
+    found = __syncthreads_or(found);
+    if (threadIdx.x == 0) hub[0] = found ? 1 : 0;
+}
+
 template <bool EPI, int FD>
 __global__ __launch_bounds__(kSpThreads) void gather_win_t(const int32_t* __restrict__ rp,
                                                           const int32_t* __restrict__ col,
@@ -379,7 +400,7 @@ __global__ __launch_bounds__(kSpThreads) void gather_win_t(const int32_t* __rest
     constexpr int WINR = kWinR;
     constexpr int HALO = kHalo;
     constexpr int CHUNK = kWinIdxChunk;
-    constexpr int FB = kFarBatch;
+    constexpr int FW = kFarWindow;
     constexpr int LPT = TC / 4;           // 8 lanes per row segment
     constexpr int RG = kSpThreads / LPT;  // 32 row groups
     constexpr int RPT = kTileR / RG;      // 2 rows per thread
@@ -445,46 +466,62 @@ __global__ __launch_bounds__(kSpThreads) void gather_win_t(const int32_t* __rest
         const int ce = min(e_hi, cs + CHUNK);
         for (int k = cs + tid; k < ce; k += kSpThreads) sidx[k - cs] = col[k];
         __syncthreads();
-        // Far entries: [row start, lb) and [le, row end) (the whole row
-        // when the window is not used), FB per row and side in flight
-        // across the thread's rows before any is added (measured faster
-        // than eight in flight per row, also on graphs with no window).
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int kb[RPT], ke[RPT];
+        // Far entries: [row start, lb) and [le, row end) of each of the
+        // thread's rows (the whole row when the window is not used), walked
+        // as one list (row 0's runs, then row 1's, ...) FW entries per round,
+        // all loaded before any is added; each row's entries are added in
+        // list order.  A hub's long list thus keeps FW loads in flight while
+        // short lists finish in one round.
+        {
+            constexpr int NSEG = 2 * RPT;
+            int seg_k[NSEG], seg_end[NSEG];  // list position -> entry index per segment
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 const int rr = g + r * RG;
-                const int a = side == 0 ? srp[rr] : (use_win ? sle[rr] : srp[rr + 1]);
-                const int b = side == 0 ? (use_win ? slb[rr] : srp[rr + 1]) : srp[rr + 1];
-                kb[r] = max(a, cs);
-                ke[r] = min(b, ce);
+                const int b0 = srp[rr], e1 = srp[rr + 1];
+                const int e0 = use_win ? slb[rr] : e1;
+                const int b1 = use_win ? sle[rr] : e1;
+                seg_k[2 * r] = max(b0, cs);
+                seg_end[2 * r] = max(min(e0, ce), seg_k[2 * r]);
+                seg_k[2 * r + 1] = max(b1, cs);
+                seg_end[2 * r + 1] = max(min(e1, ce), seg_k[2 * r + 1]);
             }
-            for (int off = 0;; off += FB) {
-                bool any = false;
-                float4 x[RPT][FB];
+            int start[NSEG + 1];
+            start[0] = 0;
 #pragma unroll
-                for (int r = 0; r < RPT; ++r)
+            for (int q2 = 0; q2 < NSEG; ++q2) start[q2 + 1] = start[q2] + (seg_end[q2] - seg_k[q2]);
+            const int total = start[NSEG];
+            for (int off = 0; off < total; off += FW) {
+                float4 x[FW];
+                int rowof[FW];
 #pragma unroll
-                    for (int u = 0; u < FB; ++u) {
-                        const int k = kb[r] + off + u;
-                        x[r][u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (k < ke[r]) {
-                            any = true;
-                            if (cq < n) x[r][u] = *reinterpret_cast<const float4*>(X + sidx[k - cs] * ldx + cq);
-                        }
+                for (int u = 0; u < FW; ++u) {
+                    const int e = off + u;
+                    x[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    rowof[u] = -1;
+                    int sg = 0;
+#pragma unroll
+                    for (int q2 = 1; q2 < NSEG; ++q2) sg += e >= start[q2] ? 1 : 0;
+                    if (e < total) {
+                        int k = 0;
+#pragma unroll
+                        for (int q2 = 0; q2 < NSEG; ++q2)
+                            if (q2 == sg) k = seg_k[q2] + (e - start[q2]);
+                        rowof[u] = sg >> 1;
+                        if (cq < n) x[u] = *reinterpret_cast<const float4*>(X + sidx[k - cs] * ldx + cq);
                     }
-                if (!any) break;
+                }
 #pragma unroll
-                for (int r = 0; r < RPT; ++r)
+                for (int u = 0; u < FW; ++u) {
 #pragma unroll
-                    for (int u = 0; u < FB; ++u)
-                        if (kb[r] + off + u < ke[r]) {
-                            acc[r].x += x[r][u].x;
-                            acc[r].y += x[r][u].y;
-                            acc[r].z += x[r][u].z;
-                            acc[r].w += x[r][u].w;
+                    for (int r = 0; r < RPT; ++r)
+                        if (rowof[u] == r) {
+                            acc[r].x += x[u].x;
+                            acc[r].y += x[u].y;
+                            acc[r].z += x[u].z;
+                            acc[r].w += x[u].w;
                         }
+                }
             }
         }
         if (!window_ready) {
@@ -715,6 +752,41 @@ int launch_rowgather(const int32_t* acol, const int32_t* arp, int n, const float
 
 bool bad_n(int64_t n) { return n <= 0 || n > (int64_t(1) << 20); }
 
+// Whether the CSR (rp, col, n) has a hub, measured once per graph (one small
+// kernel and a stream synchronisation on the first call) and remembered by
+// its device pointers.  Either gather is correct for any graph, so an entry
+// left by a freed graph whose memory a new one reuses can cost speed, never
+// a result.
+int graph_hub_cached(const int32_t* rp, const int32_t* col, int n, int32_t* dev_flag, hipStream_t s, bool& hub) {
+    struct Entry {
+        const int32_t* rp;
+        const int32_t* col;
+        int n;
+        bool hub;
+    };
+    static std::mutex mu;
+    static Entry cache[16];
+    static int next = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const Entry& e : cache)
+            if (e.rp == rp && e.col == col && e.n == n && rp) {
+                hub = e.hub;
+                return 0;
+            }
+    }
+    hipLaunchKernelGGL(graph_has_hub, dim3(1), dim3(kHubThreads), 0, s, rp, n, dev_flag);
+    if (int rc = check_launch("graph_has_hub")) return rc;
+    int32_t h = 0;
+    CG_CHECK_HIP(hipMemcpyAsync(&h, dev_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CG_CHECK_HIP(hipStreamSynchronize(s));
+    hub = h != 0;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[next] = Entry{rp, col, n, hub};
+    next = (next + 1) % 16;
+    return 0;
+}
+
 }  // namespace
 }  // namespace cg
 
@@ -787,10 +859,10 @@ static int64_t gradient_ld(int64_t n) {
 static bool gather_friendly_ld(int64_t ld) { return ld % 4 == 0 && (ld * 4) % 4096 != 0; }
 
 // Two padded n x n buffers, a copy of P (gathered by the first product) and
-// the intermediate Y, and the window runs of both adjacencies.
+// the intermediate Y, the window runs of both adjacencies and a flag.
 size_t cg_gradient_workspace(int64_t n) {
     return n > 0 ? 2 * (sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(gradient_ld(n)) + 256) +
-                       2 * (sizeof(int2) * static_cast<size_t>(n) + 256)
+                       2 * (sizeof(int2) * static_cast<size_t>(n) + 256) + 256 + sizeof(int32_t)
                  : 0;
 }
 
@@ -833,25 +905,35 @@ int gradient_impl(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* 
         ld1 = ldy;
     }
     if (!directed) {
-        // Symmetric pairs: both products through the LDS window where the
-        // tiles' neighbours are local enough (each tile decides).
+        // Symmetric pairs: each product through the LDS window, unless its
+        // adjacency has a hub (then the plain gather).
         int2* a_split = cv.take<int2>(static_cast<size_t>(n));
         int2* b_split = cv.take<int2>(static_cast<size_t>(n));
+        int32_t* flag = cv.take<int32_t>(1);
+        bool a_hub = false, b_hub = false;
+        if (int rc = graph_hub_cached(a_rowptr, a_col, ni, flag, s, a_hub)) return rc;
+        if (int rc = graph_hub_cached(b_rowptr, b_col, ni, flag, s, b_hub)) return rc;
         const unsigned sb = static_cast<unsigned>(div_up(n, kSpThreads));
-        hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, a_rowptr, a_col, ni, kHalo, a_split);
-        if (int rc = check_launch("window_split")) return rc;
-        hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, b_rowptr, b_col, ni, kHalo, b_split);
-        if (int rc = check_launch("window_split")) return rc;
         bool launched = false;
-        if (int rc = launch_gather_win<false>(a_rowptr, a_col, a_split, ni, X1, ld1, 1.0f, none, Y, ldy, s, launched))
-            return rc;
+        if (!a_hub) {
+            hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, a_rowptr, a_col, ni, kHalo, a_split);
+            if (int rc = check_launch("window_split")) return rc;
+            if (int rc = launch_gather_win<false>(a_rowptr, a_col, a_split, ni, X1, ld1, 1.0f, none, Y, ldy, s, launched))
+                return rc;
+        }
         if (!launched)
             if (int rc = launch_gather_tile<false>(a_rowptr, a_col, ni, X1, ld1, 1.0f, none, Y, ldy, s)) return rc;
         const float rl = regularizer_factor * static_cast<float>(lambda_);
         Epilogue ep{rl != 0.0f ? P : nullptr, ld_p, source_features, target_features, rl, feature_dim,
                     aligned16(P) && ld_p % 4 == 0 ? 1 : 0};
-        if (int rc = launch_gather_win<true>(b_rowptr, b_col, b_split, ni, Y, ldy, -2.0f, ep, gradient, n, s, launched))
-            return rc;
+        launched = false;
+        if (!b_hub) {
+            hipLaunchKernelGGL(window_split, dim3(sb), dim3(kSpThreads), 0, s, b_rowptr, b_col, ni, kHalo, b_split);
+            if (int rc = check_launch("window_split")) return rc;
+            if (int rc = launch_gather_win<true>(b_rowptr, b_col, b_split, ni, Y, ldy, -2.0f, ep, gradient, n, s,
+                                                 launched))
+                return rc;
+        }
         if (launched) return 0;
         return launch_gather_tile<true>(b_rowptr, b_col, ni, Y, ldy, -2.0f, ep, gradient, n, s);
     }
