@@ -679,7 +679,7 @@ def main() -> int:
     step_s = max(times)
     api_extra = api_extra_seconds(inst, P, "cuda:0")
 
-    from cugal_amd.pred import gradient_precision_for
+    from cugal_amd.pred import gradient_precision_for, mfma_density
     from cugal_amd.profile import Phase
 
     form = gradient_precision_for(inputs[0], inputs[1], "auto")
@@ -730,7 +730,7 @@ def main() -> int:
             "estimated_ratio_to_plain": (plain["seconds"] / est) if plain and "seconds" in plain else None,
             "why_not_faster_here": f"at this graph's density the two CSR gathers need ~8.6e10 reads per gradient "
                                    f"against 4 n^3 = {4.0 * args.n ** 3:.2e} MFMA flops; the matrix cores win "
-                                   f"above ~1.4 % density",
+                                   f"above ~{100 * mfma_density(args.n):.2f} % density at this n",
             "stage_errors_vs_fp64": stage_errors(inputs, P, args),
         }
         log(f"precision form: MFMA gradient {t_mfma * 1e3:.1f} ms; per-stage errors measured")

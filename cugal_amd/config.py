@@ -126,8 +126,9 @@ class Config:
     """Arithmetic of the A P B contraction: "fp32" (CSR adjacency times
     dense FP32 P), "bf16x2" (the 0/1 adjacency, exact in BF16, times P split
     into BF16 hi + lo on the matrix cores, FP32 accumulation) or "auto" (the
-    matrix cores when the adjacencies hold more than ~1.4 % of n^2
-    non-zeros, where they become the faster form, else "fp32")."""
+    matrix cores when the adjacencies hold more non-zeros than the measured
+    crossing -- 1.17 % of n^2 at n = 16384, 0.75 % at n = 32768 --, where
+    they become the faster form, else "fp32")."""
 
     node_features: str = "cugal"
     """The node features behind the distance term: "cugal" (the 4 of

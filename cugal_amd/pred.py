@@ -52,12 +52,21 @@ def _check_config(config: Config) -> None:
 
 
 # Density above which the matrix-core gradient beats the CSR gathers on one
-# MI355X.  Measured per gradient on Erdos-Renyi pairs at n = 16384: CSR
-# 7.1 / 13.8 / 21.4 / 28.4 / 39.9 ms at densities 0.4 / 0.8 / 1.2 / 1.6 /
-# 2.0 %, matrix cores 28.1-28.4 ms throughout (crossing at 1.6 %); at
-# n = 32768 the gathers cost ~4.1e-13 s per nnz * n above a 2.7 ms floor
-# and the GEMMs 205 ms, crossing near 1.3 %.
-MFMA_DENSITY = 0.014
+# MI355X, per n.  Measured per gradient on Erdos-Renyi pairs (random
+# labels, so the gathers get no window locality): n = 16384: CSR 7.2 / 14.2 /
+# 21.5 ms at densities 0.4 / 0.8 / 1.2 %, matrix cores 20.7 ms throughout
+# (crossing at 1.17 %); n = 32768: CSR 40 / 80 / 123 / 171 ms at 0.2 / 0.4 /
+# 0.6 / 0.8 %, matrix cores 159-160 ms (crossing at 0.75 %).  The gathers'
+# cost per entry grows with n as the column strips outgrow the caches, so
+# the crossing falls with n; the rule interpolates the two measurements as a
+# power of n.
+MFMA_DENSITY_16K = 0.0117
+MFMA_DENSITY_EXPONENT = 0.64
+
+
+def mfma_density(n: int) -> float:
+    """Density from which "auto" takes the matrix-core gradient."""
+    return MFMA_DENSITY_16K * (16384.0 / max(n, 1)) ** MFMA_DENSITY_EXPONENT
 
 
 def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
@@ -82,7 +91,7 @@ def gradient_precision_for(A: Adjacency, B: Adjacency, precision: str) -> str:
         return precision
     n = A.number_of_nodes()
     nnz = 0.5 * (A.col_indices.numel() + B.col_indices.numel())
-    return "bf16x2" if nnz > MFMA_DENSITY * n * n else "fp32"
+    return "bf16x2" if nnz > mfma_density(n) * n * n else "fp32"
 
 
 def _gradient_into(

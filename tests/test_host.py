@@ -167,10 +167,15 @@ def test_oracle_not_imported_by_library():
 
 def test_auto_gradient_precision_follows_density():
     # "auto" resolves on the host from the CSR sizes: the matrix cores above
-    # MFMA_DENSITY of the n^2 entries, the CSR gathers below; explicit
-    # choices pass through.
+    # mfma_density(n) of the n^2 entries, the CSR gathers below; explicit
+    # choices pass through.  The rule passes through the two measured
+    # crossings (1.17 % at n = 16384, 0.75 % at n = 32768) and falls with n.
     from cugal_amd.adjacency import Adjacency
-    from cugal_amd.pred import MFMA_DENSITY, gradient_precision_for
+    from cugal_amd.pred import gradient_precision_for, mfma_density
+
+    assert mfma_density(16384) == pytest.approx(0.0117)
+    assert mfma_density(32768) == pytest.approx(0.0075, abs=2e-4)
+    assert mfma_density(1000) > mfma_density(8192) > mfma_density(65536)
 
     n = 1000
 
@@ -179,7 +184,7 @@ def test_auto_gradient_precision_follows_density():
         rp = torch.arange(0, n * per_row + 1, per_row, dtype=torch.int32)
         return Adjacency(col, rp)
 
-    threshold = MFMA_DENSITY * n  # neighbours per row at the switch
+    threshold = mfma_density(n) * n  # neighbours per row at the switch
     sparse, dense = adj(int(threshold) - 1), adj(int(threshold) + 2)
     assert gradient_precision_for(sparse, sparse, "auto") == "fp32"
     assert gradient_precision_for(dense, dense, "auto") == "bf16x2"
