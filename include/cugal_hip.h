@@ -146,8 +146,14 @@ int cg_regularize(float* gradient, const float* P, int64_t n, int iteration, cg_
  * product gathers 128-byte segments of P's rows; with a stride that is a
  * multiple of 4 KiB those alias a few L2 sets, so such a P is first copied
  * to a row-padded buffer, while any other stride (cg_gradient_padded_ld)
- * is gathered in place.  Workspace: the padded copy and the intermediate
- * (A P)^T, n x (n + 64) FP32 each. */
+ * is gathered in place.  Each product serves the neighbours near a 64-row
+ * tile from rows staged in LDS (exact for any CSR; a row whose sorted run is
+ * not local is gathered from global memory), except for a graph with a row
+ * of more than 64 entries, which takes the plain gather: the first call with
+ * a given CSR (by its device pointers and n) synchronises the stream once to
+ * read that property, later calls do not.  Workspace: the padded copy and
+ * the intermediate (A P)^T, n x (n + 64) FP32 each, and 8 n bytes per graph
+ * for the local runs. */
 size_t cg_gradient_workspace(int64_t n);
 int64_t cg_gradient_padded_ld(int64_t n);
 int cg_gradient(const int32_t* a_col, const int32_t* a_rowptr, const int32_t* b_col, const int32_t* b_rowptr,
