@@ -126,6 +126,51 @@ def test_gradient_matches_oracle(device, sparse, n, seed, p, hub):
     np.testing.assert_allclose(out.double().cpu().numpy(), ref.numpy(), rtol=0, atol=1e-5 * scale)
 
 
+def _shuffled_rows(adj: Adjacency, seed: int) -> Adjacency:
+    """The same CSR with every row's column list in a random order."""
+    rp = adj.row_pointers.cpu().numpy()
+    col = adj.col_indices.cpu().numpy().copy()
+    rng = np.random.default_rng(seed)
+    for r in range(len(rp) - 1):
+        rng.shuffle(col[rp[r]:rp[r + 1]])
+    return Adjacency(torch.from_numpy(col).to(adj.col_indices.device), adj.row_pointers.clone())
+
+
+@pytest.mark.parametrize("n", [1024, 1000, 1001])
+@pytest.mark.parametrize("variant", ["local", "unsorted", "long_row"])
+def test_window_gradient_matches_oracle(device, n, variant):
+    # Ring lattices with 10 % shortcuts in their own order: nearly every
+    # neighbour lies within 16 rows of its 64-row tile, so both products
+    # stage their windows in LDS (n = 1000: partial tiles; n = 1001, not a
+    # multiple of 4: the plain gather).  "unsorted": every row's list
+    # shuffled (its local run cannot be found by search, so the row is
+    # gathered from global memory).  "long_row": one node joined to 100
+    # others, which sends the graph to the plain gather.
+    ga = nx.watts_strogatz_graph(n, 10, 0.1, seed=n)
+    gb = nx.watts_strogatz_graph(n, 8, 0.1, seed=n + 1)
+    if variant == "long_row":
+        ga.add_edges_from((5, j) for j in range(300, 400))
+    A = Adjacency.from_graph(ga, device)
+    B = Adjacency.from_graph(gb, device)
+    if variant == "unsorted":
+        A, B = _shuffled_rows(A, 1), _shuffled_rows(B, 2)
+    gen = torch.Generator().manual_seed(n)
+    P = torch.rand((n, n), generator=gen)
+    P /= P.sum(1, keepdim=True)
+    feats = Features(node_features(Adjacency.from_graph(ga, device), 0.5),
+                     node_features(Adjacency.from_graph(gb, device), 0.5))
+    lam = 5
+    out = torch.empty((n, n), device=device)
+    _gradient_into(out, A, B, P.to(device), feats, lam, 1.0)
+    Ad = torch.tensor(nx.to_numpy_array(ga, nodelist=range(n)))
+    Bd = torch.tensor(nx.to_numpy_array(gb, nodelist=range(n)))
+    fs, ft = feats.source.double().cpu(), feats.target.double().cpu()
+    D = torch.sqrt(((fs[:, None, :] - ft[None, :, :]) ** 2).sum(-1))
+    ref = oracle.sparse_gradient(Ad, Bd, P.double(), D, lam)
+    scale = float(ref.abs().max())
+    np.testing.assert_allclose(out.double().cpu().numpy(), ref.numpy(), rtol=0, atol=1e-5 * scale)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
 @pytest.mark.parametrize("n", [1024, 1000])
 def test_gradient_of_row_padded_P(device, precision, n):
