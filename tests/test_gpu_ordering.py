@@ -84,6 +84,12 @@ def test_locality_order_choice(device):
     assert small[0] < 0.05 and small[1] < 0.5 < 0.85 < small[2]
     assert np.array_equal(ordering.choose_position(b, "auto"), filtered)
     assert ordering.choose_position(a, "auto") is None
+    # the device's locality scores are the host counts exactly
+    for adj in (a, b):
+        arp, acol = ordering._host_csr(adj)
+        positions = [None, ordering.rcm_position(arp, acol), filtered]
+        got = ordering.device_locality_scores(adj, positions)
+        assert got == [ordering.locality_score(arp, acol, p) for p in positions]
     # the common-neighbour counts against scipy, (M M) * M entrywise
     M = nx.to_scipy_sparse_array(shuffled, nodelist=range(n), format="csr")
     C = (M @ M).multiply(M).tocoo()
