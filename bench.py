@@ -596,7 +596,7 @@ def main() -> int:
     ap.add_argument("--probe", action="store_true", help="launch each hot kernel a few times (for profilers)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--k", type=int, default=20)
     ap.add_argument("--p", type=float, default=0.1)
