@@ -241,8 +241,6 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
     __shared__ int queued[kFusedMaxRowsPerBlock];
     __shared__ float su[kFusedMaxRowsPerBlock];  // the block's incoming u2
     __shared__ int nqueued;
-    if (ctl->done) return;
-    const float mom = ctl->momentum;
     const int tid = threadIdx.x;
     float* sv = smem;
     float2* red = reinterpret_cast<float2*>(smem + round_up(n, 4));
@@ -251,17 +249,38 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
     // 1.5-2 % faster than contiguous row ranges at n = 32768).
     const int nblk = static_cast<int>(gridDim.x);
     const int rows = min(rows_per_blk, (n - static_cast<int>(blockIdx.x) + nblk - 1) / nblk);
+    const int64_t ld = n;
+    if (ctl->done) return;
+    const float mom = ctl->momentum;
     if (tid == 0) nqueued = 0;
-    for (int j = tid * 4; j < n; j += T * 4)
-        *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v2 + j);
-    for (int k = tid; k < rows; k += T) su[k] = u2[blockIdx.x + k * nblk];
+    if constexpr (T == 256) {
+        // The thread's slice of v (it reads back only its own columns) and
+        // the block's incoming u, all loads in flight at once (the 1024-
+        // thread sweep keeps the loop: the staging registers would spill).
+        float4 vg[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int col = (g * T + tid) * 4;
+            vg[g] = (FULL || col < n) ? *reinterpret_cast<const float4*>(v2 + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const float u_in = tid < rows ? u2[blockIdx.x + tid * nblk] : 0.0f;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int col = (g * T + tid) * 4;
+            if (FULL || col < n) *reinterpret_cast<float4*>(sv + col) = vg[g];
+        }
+        if (tid < rows) su[tid] = u_in;
+    } else {
+        for (int j = tid * 4; j < n; j += T * 4)
+            *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v2 + j);
+        for (int k = tid; k < rows; k += T) su[k] = u2[blockIdx.x + k * nblk];
+    }
     __syncthreads();
 
     float cs[4 * G];
 #pragma unroll
     for (int k = 0; k < 4 * G; ++k) cs[k] = 0.0f;
 
-    const int64_t ld = n;
     // One row buffer: a second one (loading row k+1 during the reduction of
     // row k) measured slower; the 16 waves' loads keep HBM busy.  `reverse`
     // walks the block's rows last to first (the column sums then add in
@@ -601,23 +620,36 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
                                                     float* __restrict__ part, int rows_per_blk,
                                                     const SinkCtl* __restrict__ ctl) {
     extern __shared__ float smem[];
-    if (ctl->done) return;
-    const float mom = ctl->momentum;
     const int tid = threadIdx.x;
     float* sv = smem;
     float* red = smem + round_up(n, 4);
-    for (int j = tid * 4; j < n; j += T * 4)
-        *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v + j);
+    const int r0 = blockIdx.x * rows_per_blk;
+    const int r1 = min(n, r0 + rows_per_blk);
+    const int64_t ld = n;
+    // The first row and the thread's slice of v (it reads back only its own
+    // columns) are loaded together before the stop flag is read, as in the
+    // LOG sweep.
+    float4 a[G];
+    if (r0 < r1) fused_load_row<T, G, false, KEEP>(a, C + r0 * ld, n);
+    float4 vg[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        vg[g] = col < n ? *reinterpret_cast<const float4*>(v + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (ctl->done) return;
+    const float mom = ctl->momentum;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int col = (g * T + tid) * 4;
+        if (col < n) *reinterpret_cast<float4*>(sv + col) = vg[g];
+    }
     __syncthreads();
     float cs[4 * G];
 #pragma unroll
     for (int k = 0; k < 4 * G; ++k) cs[k] = 0.0f;
-    const int r0 = blockIdx.x * rows_per_blk;
-    const int r1 = min(n, r0 + rows_per_blk);
-    const int64_t ld = n;
     if constexpr (4 * G * T <= 24 * 1024) {
-        float4 a[G], b[G];
-        if (r0 < r1) fused_load_row<T, G, false, KEEP>(a, C + r0 * ld, n);
+        float4 b[G];
         for (int i = r0; i < r1; i += 2) {
             if (i + 1 < r1) fused_load_row<T, G, false, KEEP>(b, C + (i + 1) * ld, n);
             std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
@@ -626,9 +658,8 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
             std_fused_row<T, G>(b, cs, i + 1, n, scale2, sv, red, u, mom);
         }
     } else {
-        float4 a[G];
         for (int i = r0; i < r1; ++i) {
-            fused_load_row<T, G, false, KEEP>(a, C + i * ld, n);
+            if (i > r0) fused_load_row<T, G, false, KEEP>(a, C + i * ld, n);
             std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
         }
     }
