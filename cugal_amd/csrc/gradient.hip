@@ -200,7 +200,7 @@ __device__ __forceinline__ void store_tile_t(const float (*tile)[TC + 1], int r0
             float pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (ep.P) {
                 if (VEC && ep.pvec && rq + 3 < n) {
-                    const float4 p4 = *reinterpret_cast<const float4*>(ep.P + pbase + rq);
+                    const float4 p4 = ld_stream4(ep.P + pbase + rq);
                     pv[0] = p4.x;
                     pv[1] = p4.y;
                     pv[2] = p4.z;
