@@ -668,8 +668,10 @@ def main() -> int:
         # step), so the K timed steps cannot crowd it out of the budget.
         if args.no_plain:
             return None
-        needs = 2.1 * step_s + 10 + alignments_left * step_s
-        if not budget.allows("plain", needs):
+        # The metric is a ratio to this leg: it may use the 30 s the optional
+        # legs leave under the wall-clock limit (they are skipped instead).
+        needs = 2.05 * step_s + 5 + alignments_left * step_s
+        if not budget.allows("plain", needs - 30):
             return budget.skipped(needs)
         e, a, pprof, (pms, pcnt), _ = run_alignment(inst, make_config(False, args, precision="fp32"), True, inputs)
         log(f"plain two-sweep: {e:.2f}s acc {a:.4f}")
