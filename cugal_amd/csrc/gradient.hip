@@ -186,45 +186,71 @@ __device__ __forceinline__ void store_tile_t(const float (*tile)[TC + 1], int r0
 #pragma unroll
         for (int e = 0; e < 4; ++e) ftj[e] = rq + e < n ? feature_row<FD>(ep.ft, rq + e) : FeatureRow{};
     }
-    for (int cc = g; cc < TC; cc += 16) {
+    // Every load of a thread's rows is issued before the store that precedes
+    // its use: loads and stores count on one counter, so a load issued after
+    // a store would also wait for that store.  P (read from HBM) is loaded
+    // for all rows at once, and so are 4-float source feature rows; an
+    // 8-float row is loaded one row ahead, after the current row's
+    // arithmetic (all of them at once would cost the kernel a wave per SIMD
+    // in registers).
+    constexpr int KR = TC / 16;
+    constexpr int FH = FD == 4 ? KR : 1;  // source feature rows held at once
+    float v[KR][4];
+    float pv[KR][4];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int cc = g + 16 * k;
         const int c = c0 + cc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[k][e] = tile[4 * q + e][cc];
+            pv[k][e] = 0.0f;
+        }
+        if (EPI && ep.P && c < n) {
+            // P read as one 16-byte load where aligned.
+            const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
+            if (VEC && ep.pvec && rq + 3 < n) {
+                const float4 p4 = ld_stream4(ep.P + pbase + rq);
+                pv[k][0] = p4.x;
+                pv[k][1] = p4.y;
+                pv[k][2] = p4.z;
+                pv[k][3] = p4.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (rq + e < n) pv[k][e] = ep.P[pbase + rq + e];
+            }
+        }
+    }
+    FeatureRow fsi[FH];
+#pragma unroll
+    for (int k = 0; k < FH; ++k) {
+        const int c = c0 + g + 16 * k;
+        fsi[k] = EPI && ep.fs && c < n ? feature_row<FD>(ep.fs, c) : FeatureRow{};
+    }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int c = c0 + g + 16 * k;
         if (c >= n) break;
-        float v[4] = {tile[4 * q + 0][cc], tile[4 * q + 1][cc], tile[4 * q + 2][cc], tile[4 * q + 3][cc]};
-        const int64_t pbase = static_cast<int64_t>(c) * ep.ldp;
         const int64_t base = static_cast<int64_t>(c) * ldo;
         if (EPI) {
             // Same terms in the same order as epi_one: scale * sum, + D,
-            // + rl (1 - 2 P); P read as one 16-byte load where aligned.
-            FeatureRow fsi{};
-            if (ep.fs) fsi = feature_row<FD>(ep.fs, c);
-            float pv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (ep.P) {
-                if (VEC && ep.pvec && rq + 3 < n) {
-                    const float4 p4 = ld_stream4(ep.P + pbase + rq);
-                    pv[0] = p4.x;
-                    pv[1] = p4.y;
-                    pv[2] = p4.z;
-                    pv[3] = p4.w;
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (rq + e < n) pv[e] = ep.P[pbase + rq + e];
-                }
-            }
+            // + rl (1 - 2 P).
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                float gv = v[e] * scale;
-                if (ep.fs) gv += feature_distance_rows<FD>(fsi, ftj[e]);
-                if (ep.P) gv += ep.rl - 2.0f * ep.rl * pv[e];
-                v[e] = gv;
+                float gv = v[k][e] * scale;
+                if (ep.fs) gv += feature_distance_rows<FD>(fsi[FH == KR ? k : 0], ftj[e]);
+                if (ep.P) gv += ep.rl - 2.0f * ep.rl * pv[k][e];
+                v[k][e] = gv;
             }
+            if (FH == 1 && ep.fs && k + 1 < KR && c + 16 < n) fsi[0] = feature_row<FD>(ep.fs, c + 16);
         }
         if (VEC && rq + 3 < n) {
-            st_stream4(out + base + rq, make_float4(v[0], v[1], v[2], v[3]));
+            st_stream4(out + base + rq, make_float4(v[k][0], v[k][1], v[k][2], v[k][3]));
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (rq + e < n) out[base + rq + e] = v[e];
+                if (rq + e < n) out[base + rq + e] = v[k][e];
         }
     }
 }
