@@ -31,6 +31,26 @@ __device__ __forceinline__ float4 ld_stream4(const float* p) {
 }
 __device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
 
+// The same 16-byte loads at 4-byte alignment, for the rows of a matrix whose
+// width is not a multiple of 4: still one dwordx4 access (global memory
+// needs dword alignment only).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ float4 ld_stream4u(const float* p) {
+    const f32x4u v = __builtin_nontemporal_load(reinterpret_cast<const f32x4u*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ld4u(const float* p) {
+    const f32x4u v = *reinterpret_cast<const f32x4u*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Elements col .. col + 3 of a length-n vector, `pad` past its end (scalar
+// loads: nothing past n is read).
+__device__ __forceinline__ float4 ld_tail4(const float* p, int col, int n, float pad) {
+    return make_float4(col < n ? p[col] : pad, col + 1 < n ? p[col + 1] : pad, col + 2 < n ? p[col + 2] : pad,
+                       col + 3 < n ? p[col + 3] : pad);
+}
+
 // Running (max, scaled sum) pair of a base-2 log-sum-exp.
 struct Lse {
     float m;

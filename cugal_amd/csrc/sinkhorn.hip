@@ -99,15 +99,28 @@ __global__ void sk_finish(const float* __restrict__ u2, const float* __restrict_
 // ---------------------------------------------------------------------------
 // FULL: n == 4 G T, every column group exists (no bounds tests).
 // KEEP: caching loads instead of non-temporal ones (see kCacheableMaxN).
+// Without FULL any width works: a row of a matrix whose width is not a
+// multiple of 4 starts at a 4-byte boundary only, so whole groups are
+// 16-byte loads at 4-byte alignment, and the group that crosses column n
+// loads its columns below n one by one (nothing past the row is read; the
+// last row ends the matrix) and holds 0 above.
 template <int T, int G, bool FULL = false, bool KEEP = false>
 __device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __restrict__ row, int n) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + threadIdx.x) * 4;
-        r[g] = (FULL || col < n) ? (KEEP ? *reinterpret_cast<const float4*>(row + col) : ld_stream4(row + col))
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (FULL)
+            r[g] = KEEP ? *reinterpret_cast<const float4*>(row + col) : ld_stream4(row + col);
+        else if (col + 3 < n)
+            r[g] = KEEP ? ld4u(row + col) : ld_stream4u(row + col);
+        else
+            r[g] = ld_tail4(row, col, n, 0.0f);
     }
 }
+
+// Row stride of the fused sweeps' column partials: whole float4 groups, so
+// the group that crosses column n stores inside its own partial row.
+__host__ __device__ constexpr int64_t part_ld(int64_t n) { return round_up(n, 4); }
 
 // Up to this width (a 576 MiB matrix) the fused sweeps read C with caching
 // loads, and the LOG sweep walks its rows backwards on every other
@@ -257,11 +270,13 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
         // The thread's slice of v (it reads back only its own columns) and
         // the block's incoming u, all loads in flight at once (the 1024-
         // thread sweep keeps the loop: the staging registers would spill).
+        // Columns from n up to the group's end hold kNegBig: their plan
+        // entries 2^(0 * scale2 + kNegBig + u) are 0.
         float4 vg[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int col = (g * T + tid) * 4;
-            vg[g] = (FULL || col < n) ? *reinterpret_cast<const float4*>(v2 + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+            vg[g] = (FULL || col + 3 < n) ? *reinterpret_cast<const float4*>(v2 + col) : ld_tail4(v2, col, n, kNegBig);
         }
         const float u_in = tid < rows ? u2[blockIdx.x + tid * nblk] : 0.0f;
 #pragma unroll
@@ -272,7 +287,8 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
         if (tid < rows) su[tid] = u_in;
     } else {
         for (int j = tid * 4; j < n; j += T * 4)
-            *reinterpret_cast<float4*>(sv + j) = *reinterpret_cast<const float4*>(v2 + j);
+            *reinterpret_cast<float4*>(sv + j) =
+                j + 3 < n ? *reinterpret_cast<const float4*>(v2 + j) : ld_tail4(v2, j, n, kNegBig);
         for (int k = tid; k < rows; k += T) su[k] = u2[blockIdx.x + k * nblk];
     }
     __syncthreads();
@@ -305,7 +321,7 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
         __syncthreads();  // queued rows may share a reduction slot
     }
 
-    float* out = part + static_cast<int64_t>(blockIdx.x) * n;
+    float* out = part + static_cast<int64_t>(blockIdx.x) * part_ld(n);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
@@ -488,7 +504,8 @@ __device__ void combine_error(double err, int want_err, int check_index, int cur
 
 template <bool PAIRS>
 __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
-    const void* __restrict__ part_raw, int nblk, int n, const float* __restrict__ v2_in, float* __restrict__ v2_out,
+    const void* __restrict__ part_raw, int nblk, int n, int64_t ldq, const float* __restrict__ v2_in,
+    float* __restrict__ v2_out,
     int raw_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
     __shared__ float rm[kCombWaves][kCombCols];
@@ -505,10 +522,10 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         m = kNegBig;
         if (j < n) {
 #pragma unroll 8
-            for (int b = wave; b < nblk; b += kCombWaves) m = fmaxf(m, part[static_cast<int64_t>(b) * n + j].x);
+            for (int b = wave; b < nblk; b += kCombWaves) m = fmaxf(m, part[static_cast<int64_t>(b) * ldq + j].x);
 #pragma unroll 8
             for (int b = wave; b < nblk; b += kCombWaves) {
-                const float2 p = part[static_cast<int64_t>(b) * n + j];
+                const float2 p = part[static_cast<int64_t>(b) * ldq + j];
                 sum += static_cast<double>(p.y) * exp2(static_cast<double>(p.x) - static_cast<double>(m));
             }
         }
@@ -519,7 +536,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         // 8.6 -> 6.5 us per combine at n = 8192.
         if (j < n) {
 #pragma unroll 32
-            for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
+            for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
         }
     }
     rm[wave][lane] = m;
@@ -635,7 +652,9 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        vg[g] = col < n ? *reinterpret_cast<const float4*>(v + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+        // Columns from n up to the group's end hold v = 0: no share of the
+        // row sums.
+        vg[g] = col + 3 < n ? *reinterpret_cast<const float4*>(v + col) : ld_tail4(v, col, n, 0.0f);
     }
     if (ctl->done) return;
     const float mom = ctl->momentum;
@@ -663,7 +682,7 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
             std_fused_row<T, G>(a, cs, i, n, scale2, sv, red, u, mom);
         }
     }
-    float* out = part + static_cast<int64_t>(blockIdx.x) * n;
+    float* out = part + static_cast<int64_t>(blockIdx.x) * part_ld(n);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
@@ -735,8 +754,8 @@ __global__ __launch_bounds__(W * kWave) void std_row_sweep(const float* __restri
 // the raw next v_j = 1 / (S_j + eps) and the column marginal v_j S_j of the
 // current (u, v).
 __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
-    const float* __restrict__ part, int nblk, int n, const float* __restrict__ v_in, float* __restrict__ v_out,
-    int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
+    const float* __restrict__ part, int nblk, int n, int64_t ldq, const float* __restrict__ v_in,
+    float* __restrict__ v_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
     __shared__ double rs[kCombWaves][kCombCols];
     if (ctl->done) return;
@@ -746,7 +765,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
     double sum = 0.0;
     if (j < n) {
 #pragma unroll 32
-        for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * n + j]);
+        for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
     }
     rs[wave][lane] = sum;
     __syncthreads();
@@ -1007,7 +1026,7 @@ size_t carve(Carver& cv, int64_t n, const Plan& p, int cap, SinkWs* w) {
     t.v2[0] = cv.take<float>(n);
     t.v2[1] = cv.take<float>(n);
     const size_t pair_bytes = sizeof(float2) * static_cast<size_t>(p.col_nblk) * n;
-    const size_t sum_bytes = p.fused ? sizeof(float) * static_cast<size_t>(p.nblk) * n : 0;
+    const size_t sum_bytes = p.fused ? sizeof(float) * static_cast<size_t>(p.nblk) * part_ld(n) : 0;
     t.part = cv.take<char>(std::max(pair_bytes, sum_bytes));
     t.errpart = cv.take<double>(p.comb_blocks);
     t.ctl = reinterpret_cast<SinkCtl*>(cv.take<char>(ctl_bytes(cap)));
@@ -1290,14 +1309,14 @@ __global__ void sk_ctl_reset(SinkCtl* __restrict__ ctl) {
 // recomputed from the same checkpoint reproduces the same bits.
 // ---------------------------------------------------------------------------
 size_t sinkhorn_segment_workspace(int64_t n) {
-    const Plan p = make_plan(n, cg_sinkhorn_fused_supported(n) && n % 4 == 0);
+    const Plan p = make_plan(n, cg_sinkhorn_fused_supported(n));
     Carver cv{nullptr, 0, ~size_t(0)};
     return carve(cv, n, p, 4, nullptr) + 256;
 }
 
 int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, const float* v_c, int steps,
                      float* u_hist, float* v_hist, void* ws, size_t ws_bytes, hipStream_t s) {
-    const int use_fused = cg_sinkhorn_fused_supported(n) && n % 4 == 0 && aligned16(C);
+    const int use_fused = cg_sinkhorn_fused_supported(n) && aligned16(C);
     const Plan p = make_plan(n, use_fused);
     Carver cv{static_cast<char*>(ws), 0, ws_bytes};
     SinkWs w{};
@@ -1311,10 +1330,11 @@ int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, cons
     auto comb = [&](bool pairs, const float* vin, float* vout) -> int {
         if (pairs)
             hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.col_nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
+                               p.col_nblk, n, static_cast<int64_t>(n), vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart,
+                               w.ctl, nullptr);
         else
             hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.nblk, n, vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
+                               p.nblk, n, part_ld(n), vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
         return check_launch("sk_combine");
     };
     if (steps <= 0) return CG_OK;
@@ -1348,7 +1368,7 @@ extern "C" {
 
 size_t cg_sinkhorn_workspace(int64_t n, int fused, int max_iterations, int eval_freq) {
     if (n <= 0) return 0;
-    const int use_fused = fused && cg_sinkhorn_fused_supported(n) && n % 4 == 0;
+    const int use_fused = fused && cg_sinkhorn_fused_supported(n);
     const Plan p = make_plan(n, use_fused);
     Carver cv{nullptr, 0, ~size_t(0)};
     return carve(cv, n, p, err_capacity(max_iterations, eval_freq), nullptr) + 256;
@@ -1370,9 +1390,10 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     if (max_iterations < 0) return fail(CG_EINVAL, "sinkhorn iterations must be >= 0");
     const int ni = static_cast<int>(n);
     const int use_fused = fused && cg_sinkhorn_fused_supported(n);
-    if (use_fused && (!aligned16(C) || n % 4 != 0)) {
-        // The fused sweep loads float4 groups: fall back to the two-sweep
-        // structure for unaligned or odd-width matrices (same results).
+    if (use_fused && !aligned16(C)) {
+        // The fused sweep loads float4 groups from a 16-byte aligned matrix
+        // (rows of any width): fall back to the two-sweep structure for an
+        // unaligned one (same results).
         return cg_sinkhorn_log_solve(C, n, regularization, u, v, max_iterations, threshold, eval_freq,
                                      momentum_start, 0, errors, errors_capacity, error_count, iterations, sweep_ms,
                                      sweep_launches, workspace, workspace_bytes, stream);
@@ -1432,12 +1453,12 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
         int* flag = check_flag(want_err);
         if (pairs)
             hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.col_nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
-                               threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
+                               p.col_nblk, ni, static_cast<int64_t>(ni), w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0,
+                               want_err, check_index, cur, threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         else
             hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
-                               p.nblk, ni, w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err, check_index, cur,
-                               threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
+                               p.nblk, ni, part_ld(ni), w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err,
+                               check_index, cur, threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         if (int rc = check_launch("sk_combine")) return rc;
         if (momentum) {
             hipLaunchKernelGGL(sk_apply_momentum, vgrid, dim3(256), 0, s, w.v2[cur], w.v2[cur ^ 1], ni, w.ctl);
@@ -1537,7 +1558,7 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
     if (eval_freq <= 0) return fail(CG_EINVAL, "sinkhorn eval_freq must be > 0");
     if (max_iterations < 0) return fail(CG_EINVAL, "sinkhorn iterations must be >= 0");
     const int ni = static_cast<int>(n);
-    const int use_fused = fused && cg_sinkhorn_fused_supported(n) && aligned16(C) && n % 4 == 0;
+    const int use_fused = fused && cg_sinkhorn_fused_supported(n) && aligned16(C);
     const Plan p = make_plan(n, use_fused);
     // Checks at 0, f, 2f, ...: at most max_iterations / f + 1 <= capacity.
     const int cap = err_capacity(max_iterations, eval_freq);
@@ -1587,9 +1608,12 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
         }
         return 0;
     };
-    auto combine = [&](int nblk, int cur, int want_err, int check_index) -> int {
+    // fused: partials of the fused sweep (rows of part_ld(n)); else of the
+    // column sweep (rows of n).
+    auto combine = [&](bool fused_parts, int cur, int want_err, int check_index) -> int {
         int* flag = check_flag(want_err);
-        hipLaunchKernelGGL(std_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, part, nblk, ni,
+        hipLaunchKernelGGL(std_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, part,
+                           fused_parts ? p.nblk : p.col_nblk, ni, fused_parts ? part_ld(ni) : static_cast<int64_t>(ni),
                            w.v2[cur], w.v2[cur ^ 1], want_err, check_index, cur, threshold, eval_freq, momentum_start,
                            w.errpart, w.ctl, flag);
         if (int rc = check_launch("std_combine")) return rc;
@@ -1607,13 +1631,13 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
     if (max_iterations > 0) {
         // v_1 from the incoming u, by the column sweep.
         if (int rc = launch_std_col(C, ni, scale2, uw, part, p, w.ctl, s)) return rc;
-        if (int rc = combine(p.col_nblk, 0, 0, -1)) return rc;
+        if (int rc = combine(false, 0, 0, -1)) return rc;
         cur = 1;
         if (use_fused) {
             for (int it = 0; it < max_iterations && !stopped; ++it) {
                 const bool chk = is_check(it);
                 if (int rc = launch_std_fused(C, ni, scale2, w.v2[cur], uw, part, p, w.ctl, s)) return rc;
-                if (int rc = combine(p.nblk, cur, chk ? 1 : 0, it)) return rc;
+                if (int rc = combine(true, cur, chk ? 1 : 0, it)) return rc;
                 final_default = cur;
                 cur ^= 1;
                 if (chk)
@@ -1626,7 +1650,7 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
                 const bool chk = is_check(it);
                 if (int rc = launch_std_row(C, ni, scale2, w.v2[cur], uw, w.ctl, s)) return rc;
                 if (int rc = launch_std_col(C, ni, scale2, uw, part, p, w.ctl, s)) return rc;
-                if (int rc = combine(p.col_nblk, cur, chk ? 1 : 0, it)) return rc;
+                if (int rc = combine(false, cur, chk ? 1 : 0, it)) return rc;
                 final_default = cur;
                 cur ^= 1;
                 if (chk)

@@ -59,7 +59,7 @@ def assert_errors_close(got, want, n, C, reg, rtol=2e-3):
     np.testing.assert_allclose(got, want, rtol=rtol, atol=error_floor(n, C, reg))
 
 
-@pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
+@pytest.mark.parametrize("n", [64, 256, 1000, 333, 379, 1001, 4096])
 @pytest.mark.parametrize("fused", [True, False])
 def test_fixed_iterations_match_oracle(device, n, fused):
     C = _cost(n, seed=n)
@@ -179,14 +179,18 @@ def test_large_marginals(device, n):
     assert abs(errors[-1] - col_err) <= 1e-2 * col_err + 1e-3
 
 
-@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 16384, 20480, 28000])
+@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 16384, 20480, 28000,
+                               2601, 5002, 7393, 8191, 12287, 12289, 20483, 32767])
 def test_fused_matches_two_sweep_at_every_block_shape(device, n):
     # The fused sweep's block holds 1 to 8 float4 groups of a row per thread
     # (256 threads up to n = 8192, 1024 above; 2600, 5000, 7000 take 3, 5
     # and 7 groups with partial last groups, 12288 and 20480 exactly 3 and 5,
     # 28000 seven), reads matrices up to n = 12288 with caching loads and
     # alternating row order and wider ones with non-temporal loads in a
-    # fixed order.  Each against the two-sweep path (pinned to the oracle at
+    # fixed order.  Widths that are not a multiple of 4 (2601 ... 32767: every
+    # residue, both block sizes, both load policies, one group short of 8 x
+    # 1024 x 4) start their rows at 4-byte boundaries and end them inside a
+    # group.  Each against the two-sweep path (pinned to the oracle at
     # small n), from a warm start so the shifted row step runs: same
     # potentials within the drift bound, same error trace.
     g = torch.Generator(device=device).manual_seed(n)

@@ -52,11 +52,13 @@ def std_error_floor(n):
     return n * 8 * 2.0 ** -24
 
 
-@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 20480])
+@pytest.mark.parametrize("n", [2600, 5000, 7000, 12288, 20480, 2601, 5002, 8191, 12289, 20483])
 def test_fused_matches_two_sweep_at_every_block_shape(device, n):
     # Fused STANDARD sweeps of 3, 5, 7 (partial) and 3, 5 (exact) float4
     # groups per thread, with caching loads up to n = 8192, against the
-    # two-sweep path: same scalings, same error trace.
+    # two-sweep path: same scalings, same error trace.  2601 ... 20483 are
+    # widths that are not a multiple of 4 (rows at 4-byte boundaries, the
+    # last group cut by n), on both block sizes and load policies.
     g = torch.Generator(device=device).manual_seed(n)
     C = torch.rand((n, n), generator=g, device=device) * 5.0
     out = {}
@@ -73,7 +75,7 @@ def test_fused_matches_two_sweep_at_every_block_shape(device, n):
     np.testing.assert_allclose(ef, et, rtol=2e-3, atol=std_error_floor(n))
 
 
-@pytest.mark.parametrize("n", [64, 256, 1000, 333, 4096])
+@pytest.mark.parametrize("n", [64, 256, 1000, 333, 1001, 4096])
 @pytest.mark.parametrize("fused", [True, False])
 def test_fixed_iterations_match_oracle(device, n, fused):
     C = _cost(n, seed=n)
