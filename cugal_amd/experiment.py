@@ -56,7 +56,7 @@ class NoiseLevel(_NoiseLevel):
 def create_graph_from_str(file: str) -> nx.Graph:
     """A graph from SNAP edge-list text: one `u v` pair per line, lines
     starting with '#' ignored, lines without exactly two integers skipped
-    (benchmark/experiment.py:155-163)."""
+    (benchmark/experiment.py:164-172)."""
     edges = [tuple(map(int, line.split())) for line in file.split("\n") if not line.startswith("#")]
     return nx.from_edgelist([edge for edge in edges if len(edge) == 2])
 

@@ -1,8 +1,10 @@
 #!/bin/sh
 # Data fixtures: SNAP-format edge lists that the reference ships with its
-# benchmark (benchmark/data/), copied unchanged so that GPU tests can run the
-# experiment harness on a real graph where the reference tree is absent.
+# benchmark (benchmark/data/), copied unchanged so that GPU tests and the
+# published-experiment bench workload can run on the real graphs where the
+# reference tree is absent.
 set -e
 cd "$(dirname "$0")"
-cp /root/reference/benchmark/data/ca-netscience.txt ca-netscience.txt
-cp /root/reference/benchmark/data/in-arenas.txt in-arenas.txt
+for f in bio-dmela ca-Erdos992 ca-GrQc ca-netscience in-arenas inf-power inf-euroroad; do
+    cp /root/reference/benchmark/data/$f.txt $f.txt
+done

@@ -319,3 +319,37 @@ def test_locality_order_choice():
     assert ordering.choose_position(small, "auto") is None
     with pytest.raises(ValueError):
         ordering.choose_position(a, "bfs")
+
+
+def test_published_experiment_fixture_and_pairs():
+    # The reference's published experiment (benchmark/results/CUGAL_res.json)
+    # as the bench's published leg reads it: its Config builds, every graph
+    # it names ships under tests/golden, and the seeded pairs are drawn in
+    # the reference's order (benchmark/experiment.py:440-460) -- the first
+    # pair of each graph is the graph itself relabelled (target, no noise)
+    # and the graph with about 10 % of its edges removed (source).
+    import json
+
+    from cugal_amd.experiment import DATA_FILES, GraphKind, NoiseLevel, generate_graph, load_edge_list
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    pub = json.load(open(os.path.join(golden, "published_results.json")))
+    exp = pub["cugal"]["experiment"]
+    cfg = Config.from_dict({**exp["algorithms"][0]["config"], "device": "cpu"})
+    assert cfg.sinkhorn_method == SinkhornMethod.LOG and cfg.hungarian_method == HungarianMethod.SCIPY
+    assert len(pub["cugal"]["entries"]) == len(exp["graphs"]) * len(exp["noise_levels"]) == 21
+    levels = [NoiseLevel(**lv) for lv in exp["noise_levels"]]
+    gen = np.random.default_rng(seed=exp["seed"])
+    sizes = {}
+    for g in exp["graphs"]:
+        kind = GraphKind[g["kind"]]
+        graph = load_edge_list(os.path.join(golden, DATA_FILES[kind]))
+        drawn = [[generate_graph(graph, None, gen, lv) for _ in range(exp["num_runs"])] for lv in levels]
+        source, target, mapping, _ = drawn[0][0]
+        assert target.number_of_edges() == graph.number_of_edges()
+        removed = 1 - source.number_of_edges() / graph.number_of_edges()
+        assert 0.04 < removed < 0.12, (kind, removed)
+        assert sorted(mapping.tolist()) == list(range(len(mapping)))
+        sizes[kind.value] = graph.number_of_nodes()
+    assert sizes == {"BIO_DMELA": 7393, "CA_ERDOS": 5094, "CA_GRQC": 4158, "CA_NETSCIENCE": 379,
+                     "IN_ARENAS": 1133, "INF_POWER": 4941, "INF_EUROROAD": 1174}
