@@ -104,6 +104,20 @@ __global__ void sk_finish(const float* __restrict__ u2, const float* __restrict_
 // 16-byte loads at 4-byte alignment, and the group that crosses column n
 // loads its columns below n one by one (nothing past the row is read; the
 // last row ends the matrix) and holds 0 above.
+// Every launch has G = ceil(n / 4T) (checked on the host), so every group
+// but a thread's last holds columns below n only: the bounds tests below
+// are on the last group alone, and the other groups' loads are issued
+// unconditionally (a branch around each would make the wave wait for every
+// load before it uses the first).
+template <int T, int G, bool FULL>
+__device__ __forceinline__ bool group_whole(int g, int n) {
+    return FULL || g + 1 < G || (g * T + static_cast<int>(threadIdx.x)) * 4 + 3 < n;
+}
+template <int T, int G, bool FULL>
+__device__ __forceinline__ bool group_live(int g, int n) {
+    return FULL || g + 1 < G || (g * T + static_cast<int>(threadIdx.x)) * 4 < n;
+}
+
 template <int T, int G, bool FULL = false, bool KEEP = false>
 __device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __restrict__ row, int n) {
 #pragma unroll
@@ -111,7 +125,7 @@ __device__ __forceinline__ void fused_load_row(float4 (&r)[G], const float* __re
         const int col = (g * T + threadIdx.x) * 4;
         if (FULL)
             r[g] = KEEP ? *reinterpret_cast<const float4*>(row + col) : ld_stream4(row + col);
-        else if (col + 3 < n)
+        else if (group_whole<T, G, false>(g, n))
             r[g] = KEEP ? ld4u(row + col) : ld_stream4u(row + col);
         else
             r[g] = ld_tail4(row, col, n, 0.0f);
@@ -144,7 +158,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        if (FULL || col < n) {
+        if (group_live<T, G, FULL>(g, n)) {
             const float4 vv = *reinterpret_cast<const float4*>(sv + col);
             r[g].x = fmaf(r[g].x, scale2, vv.x);
             r[g].y = fmaf(r[g].y, scale2, vv.y);
@@ -156,7 +170,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
     float ls = 0.0f;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if (FULL || (g * T + tid) * 4 < n)
+        if (group_live<T, G, FULL>(g, n))
             ls += (exp2_fast(r[g].x - lm) + exp2_fast(r[g].y - lm)) + (exp2_fast(r[g].z - lm) + exp2_fast(r[g].w - lm));
     }
     const Lse w = wave_lse({lm, ls});
@@ -174,7 +188,7 @@ __device__ __forceinline__ void fused_row(float4 (&r)[G], float (&cs)[4 * G], in
     if (tid == 0) u2[i] = ui;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if (FULL || (g * T + tid) * 4 < n) {
+        if (group_live<T, G, FULL>(g, n)) {
             cs[4 * g + 0] += exp2_fast(r[g].x + ui);
             cs[4 * g + 1] += exp2_fast(r[g].y + ui);
             cs[4 * g + 2] += exp2_fast(r[g].z + ui);
@@ -208,7 +222,7 @@ __device__ __forceinline__ bool fused_row_shift(float4 (&r)[G], float (&cs)[4 * 
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        if (FULL || col < n) {
+        if (group_live<T, G, FULL>(g, n)) {
             const float4 vv = *reinterpret_cast<const float4*>(sv + col);
             r[g].x = exp2_fast(fmaf(r[g].x, scale2, vv.x) + u_old);
             r[g].y = exp2_fast(fmaf(r[g].y, scale2, vv.y) + u_old);
@@ -232,7 +246,7 @@ __device__ __forceinline__ bool fused_row_shift(float4 (&r)[G], float (&cs)[4 * 
     const float f = (mom == 1.0f) ? 1.0f / S : exp2_fast(-mom * l2);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if (FULL || (g * T + tid) * 4 < n) {
+        if (group_live<T, G, FULL>(g, n)) {
             cs[4 * g + 0] = fmaf(r[g].x, f, cs[4 * g + 0]);
             cs[4 * g + 1] = fmaf(r[g].y, f, cs[4 * g + 1]);
             cs[4 * g + 2] = fmaf(r[g].z, f, cs[4 * g + 2]);
@@ -276,13 +290,14 @@ __global__ __launch_bounds__(T) void sk_fused_sweep(const float* __restrict__ C,
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int col = (g * T + tid) * 4;
-            vg[g] = (FULL || col + 3 < n) ? *reinterpret_cast<const float4*>(v2 + col) : ld_tail4(v2, col, n, kNegBig);
+            vg[g] = group_whole<T, G, FULL>(g, n) ? *reinterpret_cast<const float4*>(v2 + col)
+                                                  : ld_tail4(v2, col, n, kNegBig);
         }
         const float u_in = tid < rows ? u2[blockIdx.x + tid * nblk] : 0.0f;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int col = (g * T + tid) * 4;
-            if (FULL || col < n) *reinterpret_cast<float4*>(sv + col) = vg[g];
+            if (group_live<T, G, FULL>(g, n)) *reinterpret_cast<float4*>(sv + col) = vg[g];
         }
         if (tid < rows) su[tid] = u_in;
     } else {
@@ -602,7 +617,7 @@ __device__ __forceinline__ void std_fused_row(float4 (&r)[G], float (&cs)[4 * G]
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        if (col < n) {
+        if (group_live<T, G, false>(g, n)) {
             const float4 vv = *reinterpret_cast<const float4*>(sv + col);
             r[g].x = exp2_fast(r[g].x * scale2);
             r[g].y = exp2_fast(r[g].y * scale2);
@@ -622,7 +637,7 @@ __device__ __forceinline__ void std_fused_row(float4 (&r)[G], float (&cs)[4 * G]
     if (tid == 0) u[i] = ui;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        if ((g * T + tid) * 4 < n) {
+        if (group_live<T, G, false>(g, n)) {
             cs[4 * g + 0] = fmaf(ui, r[g].x, cs[4 * g + 0]);
             cs[4 * g + 1] = fmaf(ui, r[g].y, cs[4 * g + 1]);
             cs[4 * g + 2] = fmaf(ui, r[g].z, cs[4 * g + 2]);
@@ -654,14 +669,14 @@ __global__ __launch_bounds__(T) void std_fused_sweep(const float* __restrict__ C
         const int col = (g * T + tid) * 4;
         // Columns from n up to the group's end hold v = 0: no share of the
         // row sums.
-        vg[g] = col + 3 < n ? *reinterpret_cast<const float4*>(v + col) : ld_tail4(v, col, n, 0.0f);
+        vg[g] = group_whole<T, G, false>(g, n) ? *reinterpret_cast<const float4*>(v + col) : ld_tail4(v, col, n, 0.0f);
     }
     if (ctl->done) return;
     const float mom = ctl->momentum;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int col = (g * T + tid) * 4;
-        if (col < n) *reinterpret_cast<float4*>(sv + col) = vg[g];
+        if (group_live<T, G, false>(g, n)) *reinterpret_cast<float4*>(sv + col) = vg[g];
     }
     __syncthreads();
     float cs[4 * G];
@@ -1057,6 +1072,7 @@ hipError_t set_fused_lds(size_t lds) {
 template <int T, int G>
 int launch_fused_tg(const float* C, int n, float scale2, const float* v2, float* u2, float* part, const Plan& p,
                     const SinkCtl* ctl, hipStream_t s, int reverse) {
+    if (div_up(n, 4 * T) != G) return fail(CG_EINVAL, "fused Sinkhorn sweep: %d groups of %d threads for n=%d", G, T, n);
     const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float2) * 2 * (T / kWave);
     // The kernel also holds a small static LDS queue: ask for exactly the
     // dynamic bytes this launch needs (the attribute only grows).
@@ -1139,6 +1155,7 @@ int launch_row(const float* C, int n, int64_t ldc, float cscale, const float* ad
 template <int T, int G>
 int launch_std_fused_tg(const float* C, int n, float scale2, const float* v, float* u, float* part, const Plan& p,
                         const SinkCtl* ctl, hipStream_t s) {
+    if (div_up(n, 4 * T) != G) return fail(CG_EINVAL, "fused Sinkhorn sweep: %d groups of %d threads for n=%d", G, T, n);
     const size_t lds = sizeof(float) * round_up(n, 4) + sizeof(float) * 2 * (T / kWave);
     static size_t attr_bytes = 0;
     if (lds > static_cast<size_t>(max_lds_bytes()))
