@@ -427,6 +427,11 @@ class OracleConfig:
     hungarian_method: str = "DOUBLE_GREEDY"
     quirk_log_marginal: bool = False
     node_features: str = "cugal"  # or "fugal7": FUGAL/pred.py:24-110, simple=False
+    # Test tool, not a reference field: per-solve Sinkhorn iteration counts
+    # that replace the threshold stop (solve k runs exactly
+    # forced_iterations[k] iterations), to follow another run's stop
+    # decisions.
+    forced_iterations: list | None = None
 
 
 @dataclass
@@ -477,15 +482,17 @@ def align(source: nx.Graph, target: nx.Graph, config: OracleConfig, dtype=torch.
                 G = sparse_gradient(A, B, P, D, lam)
             else:
                 G = dense_gradient(A, B, P, D, lam)
+            forced = config.forced_iterations
+            its = config.sinkhorn_iterations if forced is None else forced[len(iterations)]
+            threshold = config.sinkhorn_threshold if forced is None else 0.0
             if config.sinkhorn_method == "LOG":
-                K, u, v, res = state.solve_log(G, config.sinkhorn_regularization, config.sinkhorn_iterations,
-                                               config.sinkhorn_threshold, config.sinkhorn_eval_freq,
-                                               config.sinkhorn_momentum_start, config.quirk_log_marginal)
+                K, u, v, res = state.solve_log(G, config.sinkhorn_regularization, its, threshold,
+                                               config.sinkhorn_eval_freq, config.sinkhorn_momentum_start,
+                                               config.quirk_log_marginal)
                 plan = plan_log(K, u, v)
             else:
-                K, u, v, res = state.solve_standard(G, config.sinkhorn_regularization, config.sinkhorn_iterations,
-                                                    config.sinkhorn_threshold, config.sinkhorn_eval_freq,
-                                                    config.sinkhorn_momentum_start)
+                K, u, v, res = state.solve_standard(G, config.sinkhorn_regularization, its, threshold,
+                                                    config.sinkhorn_eval_freq, config.sinkhorn_momentum_start)
                 plan = plan_standard(K, u, v)
             iterations.append(res.iteration_count)
             errors.append(list(res.errors))

@@ -143,29 +143,63 @@ def _oracle_run(source, target, cfg, threads=8):
         torch.set_num_threads(before)
 
 
-def _first_count_difference(device_profiles, run, n, reg, threshold, eval_freq=10):
+def _first_count_difference(device_profiles, run, states, n, reg, threshold, eval_freq=10):
     """The first solve whose iteration count differs between the device and
-    the oracle, checked to be a threshold crossing within FP32 rounding: at
-    the check where the earlier side stopped, the other side's marginal
-    error is within the FP32 floor n 2^-24 (4 + max|C| / reg log2 e) of the
-    threshold (the FP32 potentials cannot resolve the error below it)."""
+    the oracle, checked to be a threshold crossing within FP32 rounding of the
+    device's own state.  Equal counts do not mean equal states (solves that
+    run to the iteration cap drift apart unseen), so the decision is checked
+    against FP64 arithmetic from the device's captured cost and warm start:
+    at the check where the earlier side stopped, the FP64 marginal error of
+    the device's iterates is within the FP32 floor n 2^-24 (4 + max|C| / reg
+    log2 e) of the threshold on the side the device decided (at or below
+    the threshold where the device stopped, at or above where it went on)."""
     counts = [p.iteration_count for p in device_profiles]
     for k, (a, b) in enumerate(zip(counts, run.sinkhorn_iterations)):
         if a == b:
             continue
-        floor = n * 2.0 ** -24 * (4.0 + run.cost_absmax[k] / reg * 1.4426950408889634)
+        C, u0, v0 = states[k]
+        floor = n * 2.0 ** -24 * (4.0 + float(C.abs().max()) / reg * 1.4426950408889634)
         stop = min(a, b)  # the earlier stop's iteration count: its check was at iteration stop - 1
         index = (stop - 1) // eval_freq - 1  # the LOG checks are at iterations 10, 20, ...
-        other = run.sinkhorn_errors[k] if a < b else device_profiles[k].errors
-        err = other[index]
-        print(f"first differing solve {k}: device {a}, oracle {b} iterations; the other side's error at "
-              f"iteration {stop - 1} is {err:.3e} against threshold {threshold:g}, FP32 floor {floor:.2e}")
-        assert abs(err - threshold) <= floor, (k, err, floor)
+        st = oracle.SinkhornState(C.shape[0], "LOG")
+        st.u, st.v = u0.double().clone(), v0.double().clone()
+        _, _, _, res = st.solve_log(C.double(), reg, stop, 0.0, eval_freq)
+        err64 = res.errors[index]
+
+        def at(errors):
+            return f"{errors[index]:.4e}" if len(errors) > index else "not recorded"
+
+        print(f"first differing solve {k}: device {a}, oracle {b} iterations; at iteration {stop - 1} the device's "
+              f"error {at(device_profiles[k].errors)}, FP64 from the device's state {err64:.4e}, oracle's own "
+              f"{at(run.sinkhorn_errors[k])}; threshold {threshold:g}, FP32 floor {floor:.2e}")
+        if a < b:
+            assert err64 <= threshold + floor, (k, err64, floor)
+        else:
+            assert err64 >= threshold - floor, (k, err64, floor)
         return k
     return None
 
 
-def test_reference_dataset_published_configuration_within_half_a_point(device):
+def _capture_solves(monkeypatch, profiles=None):
+    """Record each LOG solve's cost and incoming potentials (on the host),
+    and with `profiles` each solve's SinkhornProfile."""
+    from cugal_amd import sinkhorn as sk
+
+    states = []
+    solve_log = sk.SinkhornState.solve_log
+
+    def spy(self, C, config, profile=None):
+        states.append((C.detach().double().cpu(), self.u.detach().double().cpu(), self.v.detach().double().cpu()))
+        out = solve_log(self, C, config, profile)
+        if profiles is not None:
+            profiles.append(profile)
+        return out
+
+    monkeypatch.setattr(sk.SinkhornState, "solve_log", spy)
+    return states
+
+
+def test_reference_dataset_published_configuration_within_half_a_point(device, monkeypatch):
     # in-arenas (1133 nodes, 5400 edges, one self-loop; the reference's
     # benchmark/data file, copied to tests/golden) with 10 % of source edges
     # removed (seed 283841238) and the published configuration, end to end:
@@ -184,6 +218,7 @@ def test_reference_dataset_published_configuration_within_half_a_point(device):
     s, t = Graph(GraphKind.IN_ARENAS, {}).get(gen, golden)
     source, target, mapping, _ = generate_graph(s, t, gen, NoiseLevel(0.1, 0.0, False))
     profile = Profile()
+    states = _capture_solves(monkeypatch)
     _, answer = cugal(copy.deepcopy(source), copy.deepcopy(target), _published_config(True), profile)
     run = _oracle_run(source, target, _published_config(False))
     acc = float(np.mean(np.array([m for _, m in answer]) == np.asarray(mapping)))
@@ -191,51 +226,79 @@ def test_reference_dataset_published_configuration_within_half_a_point(device):
     counts = [p.iteration_count for p in profile.sinkhorn_profiles]
     print(f"\nin-arenas, published configuration: accuracy {acc:.4f} vs FP64 oracle {ref_acc:.4f}; Sinkhorn "
           f"iterations {sum(counts)} vs {sum(run.sinkhorn_iterations)}")
-    first = _first_count_difference(profile.sinkhorn_profiles, run, 1133, 0.3, 1e-3)
+    first = _first_count_difference(profile.sinkhorn_profiles, run, states, 1133, 0.3, 1e-3)
     assert first is None
     assert abs(acc - ref_acc) <= 0.005 + 1e-12
 
 
-def test_harness_on_a_sensitive_reference_dataset(device):
+def test_harness_on_a_sensitive_reference_dataset(device, monkeypatch):
     # ca-netscience (379 nodes, 914 edges; the reference's benchmark/data
-    # file, copied to tests/golden) with the published configuration and
-    # the first noise level of the reference's run (10 % of source edges
-    # removed; seed 283841238), through the experiment harness.  A
-    # sensitivity check, held to 2 pp of node accuracy and of each structure
-    # score: on this instance the FP64 oracle itself moves with rounding
-    # (0.6201 on one host, 0.6016 on another with the same single thread;
-    # the FP32 CPU restatement of the same loop gives 0.6042), because many
-    # structurally equivalent nodes have nearly equal rows of Q.  Where the
-    # device and the oracle part (the first solve whose Sinkhorn iteration
-    # count differs), the stop is a threshold crossing within FP32 rounding:
-    # the FP32 floor of the marginal error (~0.009 here) is above the 1e-3
-    # threshold.  (The published 0.642 averages four instances from a build
-    # with the LOG marginal-check quirk; context, not a bound.)
+    # file, copied to tests/golden) with the published configuration and the
+    # first noise level of the reference's run (10 % of source edges removed;
+    # seed 283841238; four instances, as the published experiment averages
+    # four runs), through the experiment harness.  A sensitivity check, not
+    # the parity gate (in-arenas above holds 0.5 pp with equal counts in every
+    # solve): on this graph a large share of the rows of Q have their two
+    # largest entries within 1e-4 of each other (measured 43-45 % in the FP64
+    # oracle's Q), so rounding decides which way those rows fall, and once
+    # two runs part, their Frank-Wolfe steps settle on different
+    # near-permutations.  What the test holds, per instance: where the device
+    # and the oracle part (the first solve whose Sinkhorn count differs), the
+    # device's stop is a threshold crossing within FP32 rounding of its own
+    # iterates, against FP64 from its captured state; and the tied share of
+    # rows is large.  The FP64 oracle run with the device's own per-solve
+    # counts is also reported: its accuracy barely moves from the oracle's own
+    # (measured on 3 of 4 instances unchanged), so the accuracy gap is in
+    # the tied rows, not in the stop decisions.  Node accuracy and structure
+    # scores are held to 3 pp on the means over the four instances, the size
+    # of the measured spread (single instances part by up to 4 pp either
+    # way; the FP64 oracle alone gives 0.6201 on one host and 0.6016 on
+    # another for the first instance).  (The published 0.642 comes from a
+    # build with the LOG marginal-check quirk; context, not a bound.)
     import os
 
     golden = os.path.join(os.path.dirname(__file__), "golden")
     seed = 283841238
     level = NoiseLevel(0.1, 0.0, False)
+    runs = 4
     exp = Experiment(algorithms=[Algorithm(_published_config(True), False)],
-                     graphs=[Graph(GraphKind.CA_NETSCIENCE, {})], noise_levels=[level], seed=seed, num_runs=1,
+                     graphs=[Graph(GraphKind.CA_NETSCIENCE, {})], noise_levels=[level], seed=seed, num_runs=runs,
                      data_dir=golden)
-    result = exp.run().results[0][0][0]
+    profiles = []
+    states = _capture_solves(monkeypatch, profiles)
+    result = exp.run().results[0][0][0]  # the harness's mean over the runs
+    assert len(states) == len(profiles) == 150 * runs
 
     gen = np.random.default_rng(seed=seed)
     s, t = Graph(GraphKind.CA_NETSCIENCE, {}).get(gen, golden)
     assert (s.number_of_nodes(), s.number_of_edges()) == (379, 914)
-    source, target, mapping, _ = generate_graph(s, t, gen, level)
-    run = _oracle_run(source, target, _published_config(False))
-    answer = np.array([m for _, m in run.mapping])
-    ref_acc = float(np.mean(answer == np.asarray(mapping)))
-    ref_scores = _oracle_scores(source, target, answer, mapping)
-    print(f"\nca-netscience, 10 % source noise: accuracy {result.accuracy:.4f} (device harness) vs "
-          f"{ref_acc:.4f} (FP64 oracle); ICS/EC/S3 {result.ics:.4f}/{result.ec:.4f}/{result.sss:.4f} vs "
-          f"{ref_scores[0]:.4f}/{ref_scores[1]:.4f}/{ref_scores[2]:.4f}")
-    _first_count_difference(result.profile.sinkhorn_profiles, run, 379, 0.3, 1e-3)
-    assert abs(result.accuracy - ref_acc) <= 0.02
-    for got, want in zip((result.ics, result.ec, result.sss), ref_scores):
-        assert abs(got - want) <= 0.02
+    pairs = [generate_graph(s, t, gen, level) for _ in range(runs)]
+    ref_accs, forced_accs, ref_scores = [], [], []
+    for r, (source, target, mapping, _) in enumerate(pairs):
+        k = slice(150 * r, 150 * (r + 1))
+        run = _oracle_run(source, target, _published_config(False))
+        answer = np.array([m for _, m in run.mapping])
+        ref_accs.append(float(np.mean(answer == np.asarray(mapping))))
+        ref_scores.append(_oracle_scores(source, target, answer, mapping))
+        _first_count_difference(profiles[k], run, states[k], 379, 0.3, 1e-3)
+        counts = [p.iteration_count for p in profiles[k]]
+        forced = _oracle_run(source, target, _published_config(False, forced_iterations=counts))
+        assert forced.sinkhorn_iterations == counts
+        forced_accs.append(float(np.mean(np.array([m for _, m in forced.mapping]) == np.asarray(mapping))))
+        top2 = forced.P.topk(2, dim=1).values
+        tied = float(((top2[:, 0] - top2[:, 1]) <= 1e-4 * top2[:, 0]).double().mean())
+        print(f"\ninstance {r}: FP64 oracle {ref_accs[-1]:.4f}, with the device's counts {forced_accs[-1]:.4f}; "
+              f"rows of its Q tied within 1e-4: {tied:.1%}")
+        assert tied >= 0.25
+    ref_acc = float(np.mean(ref_accs))
+    ref_mean = np.mean(np.array(ref_scores, dtype=np.float64), axis=0)
+    print(f"ca-netscience, 10 % source noise, mean of {runs} instances: accuracy {result.accuracy:.4f} (device "
+          f"harness) vs {ref_acc:.4f} (FP64 oracle; with the device's counts {np.mean(forced_accs):.4f}); "
+          f"ICS/EC/S3 {result.ics:.4f}/{result.ec:.4f}/{result.sss:.4f} vs "
+          f"{ref_mean[0]:.4f}/{ref_mean[1]:.4f}/{ref_mean[2]:.4f}")
+    assert abs(result.accuracy - ref_acc) <= 0.03
+    for got, want in zip((result.ics, result.ec, result.sss), ref_mean):
+        assert abs(got - want) <= 0.03
     assert result.profile.frank_wolfe_iterations == 150
 
 

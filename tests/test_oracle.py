@@ -475,6 +475,30 @@ def test_pipeline_recovers_isomorphic_pair():
     assert alignment_accuracy(inst.source_mapping, answer_columns(run.mapping)) == 1.0
 
 
+def test_forced_iterations_follow_a_run():
+    # The oracle given a run's own per-solve Sinkhorn counts (threshold
+    # replaced by the counts) reproduces that run bit for bit; a count
+    # changed in one solve changes the result.
+    import copy
+
+    from cugal_amd.synthetic import erdos_renyi
+
+    inst = erdos_renyi(50, 0.1, 0.05, seed=3)
+    cfg = dict(sinkhorn_regularization=0.3, sinkhorn_method="LOG", sinkhorn_threshold=1e-3, mu=2.0, iter_count=3,
+               frank_wolfe_iter_count=3, use_sparse_adjacency=True)
+    free = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target), oracle.OracleConfig(**cfg))
+    assert len(set(free.sinkhorn_iterations)) > 1  # stops at different counts
+    forced = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target),
+                          oracle.OracleConfig(**cfg, forced_iterations=list(free.sinkhorn_iterations)))
+    assert forced.sinkhorn_iterations == free.sinkhorn_iterations
+    assert torch.equal(forced.P, free.P)
+    counts = list(free.sinkhorn_iterations)
+    counts[2] += 1
+    other = oracle.align(copy.deepcopy(inst.source), copy.deepcopy(inst.target),
+                         oracle.OracleConfig(**cfg, forced_iterations=counts))
+    assert other.sinkhorn_iterations == counts and not torch.equal(other.P, free.P)
+
+
 def test_pipeline_default_config_small():
     # Reference default Config (STANDARD, FP64, dense quirk, DOUBLE_GREEDY).
     from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi
