@@ -229,7 +229,7 @@ int cg_sinkhorn_log_marginal(const float* K, int64_t n, const float* u, const fl
  * stops when it is below `threshold`.  momentum_start < 0 disables Lehmann
  * momentum (cugal/sinkhorn.py:78-88); otherwise it is the reference's
  * sinkhorn_momentum_start.  fused = 1 reads C once per iteration (n <=
- * 32768, n % 4 == 0, 16-byte aligned C; otherwise the two-sweep structure is
+ * 32768, any width, 16-byte aligned C; otherwise the two-sweep structure is
  * used, same results); fused = 0 reads it twice (column then row sweep).
  * Synchronises the stream once at the end.  Host outputs: errors[0 ..
  * min(count, errors_capacity)), *error_count, *iterations (the reference's
