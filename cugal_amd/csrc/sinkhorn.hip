@@ -463,6 +463,12 @@ __device__ double lehmann_momentum(const double* errors, int nerr, int eval_freq
 // ---------------------------------------------------------------------------
 constexpr int kCombCols = 64;
 constexpr int kCombWaves = 4;
+// Waves per combine block: 16 where the columns make few blocks (n <= 2048,
+// at most 32 blocks), so more partial rows are in flight (n = 1174: 18.0 ->
+// 14.5 us per LOG iteration); 4 above, where 16 measured equal or slower
+// (32768: 637 against 630 us).
+constexpr int kCombWavesNarrow = 16;
+constexpr int64_t kCombNarrowMaxN = 2048;
 
 // The error bookkeeping shared by the combines: wave 0 of each block holds
 // the per-column error terms; the blocks' FP64 sums are totalled in fixed
@@ -470,10 +476,11 @@ constexpr int kCombWaves = 4;
 // below the threshold and otherwise updates the Lehmann factor.  It also
 // stores the stop decision into `host_flag` (pinned host memory, written at
 // system scope) when given, so the host polls a flag instead of copying one.
+template <int W>
 __device__ void combine_error(double err, int want_err, int check_index, int cur_buf, double threshold,
                               int eval_freq, int momentum_start, double* __restrict__ errpart,
                               SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
-    __shared__ double dred[kCombWaves];
+    __shared__ double dred[W];
     __shared__ int is_last;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
@@ -500,7 +507,7 @@ __device__ void combine_error(double err, int want_err, int check_index, int cur
     __syncthreads();
     if (threadIdx.x == 0) {
         double sum = 0.0;
-        for (int w = 0; w < kCombWaves; ++w) sum += dred[w];
+        for (int w = 0; w < W; ++w) sum += dred[w];
         double* errors = ctl_errors(ctl);
         errors[ctl->nerr] = sum;
         ctl->nerr += 1;
@@ -517,14 +524,14 @@ __device__ void combine_error(double err, int want_err, int check_index, int cur
     }
 }
 
-template <bool PAIRS>
-__global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
+template <bool PAIRS, int W>
+__global__ __launch_bounds__(kCombCols* W) void sk_combine(
     const void* __restrict__ part_raw, int nblk, int n, int64_t ldq, const float* __restrict__ v2_in,
     float* __restrict__ v2_out,
     int raw_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
-    __shared__ float rm[kCombWaves][kCombCols];
-    __shared__ double rs[kCombWaves][kCombCols];
+    __shared__ float rm[W][kCombCols];
+    __shared__ double rs[W][kCombCols];
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
@@ -537,9 +544,9 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         m = kNegBig;
         if (j < n) {
 #pragma unroll 8
-            for (int b = wave; b < nblk; b += kCombWaves) m = fmaxf(m, part[static_cast<int64_t>(b) * ldq + j].x);
+            for (int b = wave; b < nblk; b += W) m = fmaxf(m, part[static_cast<int64_t>(b) * ldq + j].x);
 #pragma unroll 8
-            for (int b = wave; b < nblk; b += kCombWaves) {
+            for (int b = wave; b < nblk; b += W) {
                 const float2 p = part[static_cast<int64_t>(b) * ldq + j];
                 sum += static_cast<double>(p.y) * exp2(static_cast<double>(p.x) - static_cast<double>(m));
             }
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         // 8.6 -> 6.5 us per combine at n = 8192.
         if (j < n) {
 #pragma unroll 32
-            for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
+            for (int b = wave; b < nblk; b += W) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
         }
     }
     rm[wave][lane] = m;
@@ -562,21 +569,21 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void sk_combine(
         double L;
         if (PAIRS) {
             float M = kNegBig;
-            for (int w = 0; w < kCombWaves; ++w) M = fmaxf(M, rm[w][lane]);
+            for (int w = 0; w < W; ++w) M = fmaxf(M, rm[w][lane]);
             double S = 0.0;
-            for (int w = 0; w < kCombWaves; ++w)
+            for (int w = 0; w < W; ++w)
                 S += rs[w][lane] * exp2(static_cast<double>(rm[w][lane]) - static_cast<double>(M));
             L = static_cast<double>(M) + log2(S);
         } else {
             double S = 0.0;
-            for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+            for (int w = 0; w < W; ++w) S += rs[w][lane];
             L = log2(S);
         }
         const double vj = static_cast<double>(v2_in[j]);
         v2_out[j] = static_cast<float>(raw_out ? (L - vj) : (vj - L));
         if (want_err) err = fabs(exp2(L) - 1.0);
     }
-    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
+    combine_error<W>(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
 }
 
 // v_next = (1 - m) v - m * lse  (only when Lehmann momentum is configured).
@@ -768,11 +775,12 @@ __global__ __launch_bounds__(W * kWave) void std_row_sweep(const float* __restri
 // Combine: column sums S_j = sum_i u_i K_ij (FP64 over the partials) give
 // the raw next v_j = 1 / (S_j + eps) and the column marginal v_j S_j of the
 // current (u, v).
-__global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
+template <int W>
+__global__ __launch_bounds__(kCombCols* W) void std_combine(
     const float* __restrict__ part, int nblk, int n, int64_t ldq, const float* __restrict__ v_in,
     float* __restrict__ v_out, int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
-    __shared__ double rs[kCombWaves][kCombCols];
+    __shared__ double rs[W][kCombCols];
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
@@ -780,18 +788,18 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void std_combine(
     double sum = 0.0;
     if (j < n) {
 #pragma unroll 32
-        for (int b = wave; b < nblk; b += kCombWaves) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
+        for (int b = wave; b < nblk; b += W) sum += static_cast<double>(part[static_cast<int64_t>(b) * ldq + j]);
     }
     rs[wave][lane] = sum;
     __syncthreads();
     double err = 0.0;
     if (wave == 0 && j < n) {
         double S = 0.0;
-        for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+        for (int w = 0; w < W; ++w) S += rs[w][lane];
         v_out[j] = static_cast<float>(1.0 / (S + static_cast<double>(kStdEps)));
         if (want_err) err = fabs(static_cast<double>(v_in[j]) * S - 1.0);
     }
-    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
+    combine_error<W>(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
 }
 
 // v_next = v^(1-m) * raw^m once the check of this combine has set m.
@@ -916,11 +924,12 @@ __global__ __launch_bounds__(W * kWave) void std64_row_sweep(const double* __res
     if (lane == 0) u[i] = std_blend64(u[i], 1.0 / (s + kStdEps64), mom);
 }
 
-__global__ __launch_bounds__(kCombCols* kCombWaves) void std64_combine(
+template <int W>
+__global__ __launch_bounds__(kCombCols* W) void std64_combine(
     const double* __restrict__ part, int nblk, int n, const double* __restrict__ v_in, double* __restrict__ v_out,
     int want_err, int check_index, int cur_buf, double threshold, int eval_freq, int momentum_start,
     double* __restrict__ errpart, SinkCtl* __restrict__ ctl, int* __restrict__ host_flag) {
-    __shared__ double rs[kCombWaves][kCombCols];
+    __shared__ double rs[W][kCombCols];
     if (ctl->done) return;
     const int lane = threadIdx.x & (kCombCols - 1);
     const int wave = threadIdx.x / kCombCols;
@@ -928,18 +937,18 @@ __global__ __launch_bounds__(kCombCols* kCombWaves) void std64_combine(
     double sum = 0.0;
     if (j < n) {
 #pragma unroll 32
-        for (int b = wave; b < nblk; b += kCombWaves) sum += part[static_cast<int64_t>(b) * n + j];
+        for (int b = wave; b < nblk; b += W) sum += part[static_cast<int64_t>(b) * n + j];
     }
     rs[wave][lane] = sum;
     __syncthreads();
     double err = 0.0;
     if (wave == 0 && j < n) {
         double S = 0.0;
-        for (int w = 0; w < kCombWaves; ++w) S += rs[w][lane];
+        for (int w = 0; w < W; ++w) S += rs[w][lane];
         v_out[j] = 1.0 / (S + kStdEps64);
         if (want_err) err = fabs(v_in[j] * S - 1.0);
     }
-    combine_error(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
+    combine_error<W>(err, want_err, check_index, cur_buf, threshold, eval_freq, momentum_start, errpart, ctl, host_flag);
 }
 
 __global__ void std64_apply_momentum(const double* __restrict__ v_in, double* __restrict__ v_io, int n,
@@ -1021,6 +1030,32 @@ Plan make_plan(int64_t n, int fused) {
     p.col_nblk = static_cast<int>(div_up(n, p.col_rows));
     p.comb_blocks = static_cast<int>(div_up(n, kCombCols));
     return p;
+}
+
+// The combines with the wave count their width calls for.
+template <bool PAIRS, typename... Args>
+void launch_sk_combine(int64_t n, int blocks, hipStream_t s, Args... args) {
+    if (n <= kCombNarrowMaxN)
+        hipLaunchKernelGGL((sk_combine<PAIRS, kCombWavesNarrow>), dim3(blocks), dim3(kCombCols * kCombWavesNarrow), 0,
+                           s, args...);
+    else
+        hipLaunchKernelGGL((sk_combine<PAIRS, kCombWaves>), dim3(blocks), dim3(kCombCols * kCombWaves), 0, s, args...);
+}
+template <typename... Args>
+void launch_std_combine(int64_t n, int blocks, hipStream_t s, Args... args) {
+    if (n <= kCombNarrowMaxN)
+        hipLaunchKernelGGL((std_combine<kCombWavesNarrow>), dim3(blocks), dim3(kCombCols * kCombWavesNarrow), 0, s,
+                           args...);
+    else
+        hipLaunchKernelGGL((std_combine<kCombWaves>), dim3(blocks), dim3(kCombCols * kCombWaves), 0, s, args...);
+}
+template <typename... Args>
+void launch_std64_combine(int64_t n, int blocks, hipStream_t s, Args... args) {
+    if (n <= kCombNarrowMaxN)
+        hipLaunchKernelGGL((std64_combine<kCombWavesNarrow>), dim3(blocks), dim3(kCombCols * kCombWavesNarrow), 0, s,
+                           args...);
+    else
+        hipLaunchKernelGGL((std64_combine<kCombWaves>), dim3(blocks), dim3(kCombCols * kCombWaves), 0, s, args...);
 }
 
 size_t ctl_bytes(int err_capacity) { return sizeof(SinkCtl) + sizeof(double) * static_cast<size_t>(err_capacity); }
@@ -1346,11 +1381,11 @@ int sinkhorn_segment(const float* C, int n, float scale2, const float* u_c, cons
     if (int rc = check_launch("sk_ctl_reset")) return rc;
     auto comb = [&](bool pairs, const float* vin, float* vout) -> int {
         if (pairs)
-            hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+            launch_sk_combine<true>(n, p.comb_blocks, s, w.part,
                                p.col_nblk, n, static_cast<int64_t>(n), vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart,
                                w.ctl, nullptr);
         else
-            hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+            launch_sk_combine<false>(n, p.comb_blocks, s, w.part,
                                p.nblk, n, part_ld(n), vin, vout, 0, 0, -1, 0, 0.0, 1, -1, w.errpart, w.ctl, nullptr);
         return check_launch("sk_combine");
     };
@@ -1469,11 +1504,11 @@ int cg_sinkhorn_log_solve(const float* C, int64_t n, float regularization, float
     auto combine = [&](bool pairs, int cur, int want_err, int check_index) -> int {
         int* flag = check_flag(want_err);
         if (pairs)
-            hipLaunchKernelGGL(sk_combine<true>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+            launch_sk_combine<true>(n, p.comb_blocks, s, w.part,
                                p.col_nblk, ni, static_cast<int64_t>(ni), w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0,
                                want_err, check_index, cur, threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         else
-            hipLaunchKernelGGL(sk_combine<false>, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+            launch_sk_combine<false>(n, p.comb_blocks, s, w.part,
                                p.nblk, ni, part_ld(ni), w.v2[cur], w.v2[cur ^ 1], momentum ? 1 : 0, want_err,
                                check_index, cur, threshold, eval_freq, momentum_start, w.errpart, w.ctl, flag);
         if (int rc = check_launch("sk_combine")) return rc;
@@ -1629,7 +1664,7 @@ int cg_sinkhorn_standard_solve(const float* C, int64_t n, float regularization, 
     // column sweep (rows of n).
     auto combine = [&](bool fused_parts, int cur, int want_err, int check_index) -> int {
         int* flag = check_flag(want_err);
-        hipLaunchKernelGGL(std_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, part,
+        launch_std_combine(n, p.comb_blocks, s, part,
                            fused_parts ? p.nblk : p.col_nblk, ni, fused_parts ? part_ld(ni) : static_cast<int64_t>(ni),
                            w.v2[cur], w.v2[cur ^ 1], want_err, check_index, cur, threshold, eval_freq, momentum_start,
                            w.errpart, w.ctl, flag);
@@ -1780,7 +1815,7 @@ int cg_sinkhorn_standard_solve_f64(const double* C, int64_t n, double regulariza
             g_done_slots[check_slot] = 0;
             flag = g_done_slots + check_slot;
         }
-        hipLaunchKernelGGL(std64_combine, dim3(p.comb_blocks), dim3(kCombCols * kCombWaves), 0, s, w.part,
+        launch_std64_combine(n, p.comb_blocks, s, w.part,
                            p.col_nblk, ni, w.v[cur], w.v[cur ^ 1], want_err, check_index, cur, threshold, eval_freq,
                            momentum_start, w.errpart, w.ctl, flag);
         if (int rc = check_launch("std64_combine")) return rc;
