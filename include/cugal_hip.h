@@ -228,7 +228,7 @@ int cg_sinkhorn_log_marginal(const float* K, int64_t n, const float* u, const fl
  * iteration 0) the column-marginal L1 error is recorded (FP64) and the solve
  * stops when it is below `threshold`.  momentum_start < 0 disables Lehmann
  * momentum (cugal/sinkhorn.py:78-88); otherwise it is the reference's
- * sinkhorn_momentum_start.  fused = 1 reads C once per iteration (n <=
+ * sinkhorn_momentum_start.  fused = 1 reads C once per iteration (4 <= n <=
  * 32768, any width, 16-byte aligned C; otherwise the two-sweep structure is
  * used, same results); fused = 0 reads it twice (column then row sweep).
  * Synchronises the stream once at the end.  Host outputs: errors[0 ..
