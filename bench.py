@@ -999,10 +999,13 @@ def published_experiment_leg(runs: int = 4, forced: bool = True, budget=None) ->
     published cuGAL mean time and accuracy.  At this threshold the solves
     of the five larger graphs run to the 500-iteration cap, the published
     ones and this build's alike; the two small graphs stop earlier.  The
-    published runs also carry the LOG marginal check that rewrites K in
-    place (cugal/sinkhorn.py:67-74, called at :112-113), which this build
-    does not repeat, so node accuracies are context, not a parity test (that
-    is against the FP64 oracle).  With `forced`, one run per graph is
+    published accuracies and iteration counts come from a source version
+    other than the current one (the FP64 oracle of the current source lands
+    where this build does on these pairs, and with the current source's
+    in-place LOG marginal check, cugal/sinkhorn.py:67-74, no solve
+    converges), so they are context; parity on these pairs is held against
+    the oracle's alignment of them (tests/golden/published_pairs_oracle.json).
+    With `forced`, one run per graph is
     repeated with the threshold at 0, so that every solve runs 500
     iterations, and the Sinkhorn time per iteration compares directly with
     the published Sinkhorn time over its iteration count."""

@@ -10,7 +10,8 @@ Reference quirks (see the `quirks` arguments):
     since it is the reference's result for use_sparse_adjacency=False;
   * the LOG solver's marginal check overwrites K with exp(K+u+v)-u-v
     (cugal/sinkhorn.py:67-74): off by default (quirk_log_marginal=True
-    reproduces it);
+    reproduces it in the CPU branch, "gpu" in the CUDA branch, whose
+    v-update reads a copy of K taken before the loop);
   * DOUBLE_GREEDY's argsort is not stable (cugal/greedy_lap.py:52): a stable
     sort is used, identical whenever the row maxima are distinct;
   * a self-loop counts once in the dense adjacency and twice in the sparse
@@ -282,15 +283,21 @@ class SinkhornState:
         return torch.sum(abs(torch.sum(plan, dim=0) - 1)).item()
 
     def solve_log(self, C, reg, iterations, threshold, eval_freq, momentum_start=None, quirk_log_marginal=False):
-        """cugal/sinkhorn.py:90-120 (CPU branch: torch.logsumexp)."""
+        """cugal/sinkhorn.py:90-120 (CPU branch: torch.logsumexp).
+        quirk_log_marginal: False, True (the CPU branch with the in-place
+        marginal check), or "gpu": the CUDA branch's version of it, where the
+        v-update reads the transposed copy of K taken before the loop
+        (:99-100, :104), which the check never touches, and only the
+        u-update and the returned K see the rewritten K."""
         K = -C / reg
+        K_v = K.clone() if quirk_log_marginal == "gpu" else K
         errors, momentum = [], 1.0
         iteration = -1
         for iteration in range(iterations):
-            self.v = self.v * (1 - momentum) - momentum * torch.logsumexp(K + self.u[:, None], 0)
+            self.v = self.v * (1 - momentum) - momentum * torch.logsumexp(K_v + self.u[:, None], 0)
             self.u = self.u * (1 - momentum) - momentum * torch.logsumexp(K + self.v[None, :], 1)
             if iteration % eval_freq == 0 and iteration != 0:
-                errors.append(self.marginal_error_log(K, quirk_log_marginal))
+                errors.append(self.marginal_error_log(K, bool(quirk_log_marginal)))
                 if errors[-1] < threshold:
                     break
                 momentum = self.lehmann_momentum(momentum_start, eval_freq, errors)
@@ -425,7 +432,7 @@ class OracleConfig:
     use_sparse_adjacency: bool = False
     sinkhorn_momentum_start: int | None = None
     hungarian_method: str = "DOUBLE_GREEDY"
-    quirk_log_marginal: bool = False
+    quirk_log_marginal: bool | str = False  # False, True or "gpu" (SinkhornState.solve_log)
     node_features: str = "cugal"  # or "fugal7": FUGAL/pred.py:24-110, simple=False
     # Test tool, not a reference field: per-solve Sinkhorn iteration counts
     # that replace the threshold stop (solve k runs exactly

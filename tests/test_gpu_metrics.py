@@ -373,3 +373,61 @@ def test_sensitive_dataset_quasi_permutation_matches_oracle(device):
     rel = float(np.abs(P - R).max() / R.max())
     print(f"\nca-netscience, fixed iterations: max |Q - Q_FP64| / max Q = {rel:.2e}")
     np.testing.assert_allclose(P, R, rtol=1e-4, atol=1e-6 * float(R.max()))
+
+
+def test_published_pairs_match_the_oracle_fixture(device, monkeypatch):
+    # The reference's published experiment pairs (the shipped graphs, seed
+    # 283841238, drawn in the order of benchmark/experiment.py:440-460) at
+    # source noise 0.1 with the published Config, against the FP64 oracle's
+    # alignment of the same pairs (tests/golden/published_pairs_oracle.json,
+    # written by tests/golden/published_pairs_oracle.py).  The published
+    # accuracies themselves are not a bar: they come from a source version
+    # other than the one the oracle restates (with the in-place marginal
+    # check of cugal/sinkhorn.py:67-74 no solve converges; see the fixture's
+    # "quirk" entry).  Per graph, over the four runs: the Sinkhorn iteration
+    # total within 1 % and the mean node accuracy within 0.5 pp; in every run
+    # whose counts differ, the first differing solve is a threshold crossing
+    # within FP32 rounding of the device's own state.
+    import copy
+    import importlib.util
+    import os
+    from types import SimpleNamespace
+
+    from cugal_amd import cugal
+    from cugal_amd.profile import Profile
+    from cugal_amd.synthetic import alignment_accuracy, answer_columns
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    spec = importlib.util.spec_from_file_location("published_pairs_oracle",
+                                                  os.path.join(golden, "published_pairs_oracle.py"))
+    gen_mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen_mod)
+    fixture = json.load(open(os.path.join(golden, "published_pairs_oracle.json")))["graphs"]
+    pairs = gen_mod.published_pairs()
+    states = _capture_solves(monkeypatch)
+    checked = 0
+    for name, runs in sorted(fixture.items()):
+        if name.endswith(" quirk"):
+            continue
+        accs, refs, its, ref_its = [], [], 0, 0
+        for r, want in sorted(runs.items()):
+            source, target, mapping, _ = pairs[name][int(r)]
+            profile = Profile()
+            states.clear()
+            _, answer = cugal(copy.deepcopy(source), copy.deepcopy(target), _published_config(True), profile)
+            run = SimpleNamespace(sinkhorn_iterations=want["sinkhorn_iterations"],
+                                  sinkhorn_errors=[[] for _ in want["sinkhorn_iterations"]])
+            n = max(source.number_of_nodes(), target.number_of_nodes())
+            _first_count_difference(profile.sinkhorn_profiles, run, states, n, 0.3, 1e-3)
+            accs.append(alignment_accuracy(mapping, answer_columns(answer)))
+            refs.append(want["accuracy"])
+            its += sum(p.iteration_count for p in profile.sinkhorn_profiles)
+            ref_its += sum(want["sinkhorn_iterations"])
+        acc, ref = float(np.mean(accs)), float(np.mean(refs))
+        print(f"\n{name} published pairs ({len(accs)} runs): accuracy {acc:.4f} vs FP64 oracle {ref:.4f} "
+              f"(per run {np.round(accs, 4).tolist()} vs {np.round(refs, 4).tolist()}); Sinkhorn iterations "
+              f"{its} vs {ref_its}")
+        assert abs(its - ref_its) <= 0.01 * ref_its
+        assert abs(acc - ref) <= 0.005 + 1e-12
+        checked += 1
+    assert checked >= 1

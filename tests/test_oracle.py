@@ -499,6 +499,34 @@ def test_forced_iterations_follow_a_run():
     assert other.sinkhorn_iterations == counts and not torch.equal(other.P, free.P)
 
 
+def test_log_marginal_quirk_forms():
+    # The in-place LOG marginal check (cugal/sinkhorn.py:67-74) first runs
+    # at iteration 10, so through 10 iterations every form is the clean
+    # solve.  After it, the CUDA branch's form ("gpu") still takes the
+    # v-update from the untouched K (the copy of :99-100): v after 12
+    # iterations is -LSE_i(K0 + u after 11); the CPU branch's form (True)
+    # reads the rewritten K there and departs from that.
+    gen = torch.Generator().manual_seed(5)
+    C = torch.rand(12, 12, generator=gen, dtype=torch.float64)
+    K0 = -C / 0.3
+
+    def run(quirk, its):
+        st = oracle.SinkhornState(12, "LOG")
+        _, u, v, res = st.solve_log(C.clone(), 0.3, its, 0.0, 10, quirk_log_marginal=quirk)
+        return u.clone(), v.clone(), res
+
+    clean = run(False, 10)
+    for quirk in (True, "gpu"):
+        u, v, _ = run(quirk, 10)
+        assert torch.equal(u, clean[0]) and torch.equal(v, clean[1])
+    for quirk, follows in ((True, False), ("gpu", True)):
+        u11, _, res = run(quirk, 11)
+        _, v12, _ = run(quirk, 12)
+        assert len(res.errors) == 1 and res.errors[0] > 1.0  # the check's error carries -u - v
+        want = -torch.logsumexp(K0 + u11[:, None], 0)
+        assert torch.allclose(v12, want, rtol=0, atol=1e-12) is follows
+
+
 def test_pipeline_default_config_small():
     # Reference default Config (STANDARD, FP64, dense quirk, DOUBLE_GREEDY).
     from cugal_amd.synthetic import alignment_accuracy, answer_columns, erdos_renyi
