@@ -431,3 +431,42 @@ def test_published_pairs_match_the_oracle_fixture(device, monkeypatch):
         assert abs(acc - ref) <= 0.005 + 1e-12
         checked += 1
     assert checked >= 1
+
+
+def test_published_pair_at_full_width_quasi_permutation_matches_oracle(device):
+    # bio-dmela, the largest published graph (n = 7393: an odd width, so the
+    # fused sweeps take their any-width path), published pair 0 at the
+    # published regularisation and mu, two Frank-Wolfe steps of 20 Sinkhorn
+    # iterations each (fixed counts: threshold 0): Q entry by entry against
+    # the FP64 oracle, held to the in-arenas bound (|C|/reg reaches a few
+    # hundred, so each FP32 exponent carries up to ~5e-5 relative rounding).
+    import copy
+    import importlib.util
+    import os
+
+    from cugal_amd import cugal
+    from cugal_amd.profile import Profile
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    spec = importlib.util.spec_from_file_location("published_pairs_oracle",
+                                                  os.path.join(golden, "published_pairs_oracle.py"))
+    gen_mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen_mod)
+    source, target, _, _ = gen_mod.published_pairs()["BIO_DMELA"][0]
+    assert max(source.number_of_nodes(), target.number_of_nodes()) == 7393
+    cfg = Config(device="cuda:0", dtype=torch.float32, sinkhorn_method=SinkhornMethod.LOG,
+                 sinkhorn_regularization=0.3, sinkhorn_iterations=20, sinkhorn_threshold=0.0, mu=2.0,
+                 iter_count=1, frank_wolfe_iter_count=2, use_sparse_adjacency=True,
+                 hungarian_method=HungarianMethod.SCIPY)
+    ref = oracle.OracleConfig(sinkhorn_regularization=0.3, sinkhorn_method="LOG", sinkhorn_threshold=0.0, mu=2.0,
+                              iter_count=1, frank_wolfe_iter_count=2, use_sparse_adjacency=True,
+                              hungarian_method="SCIPY")
+    ref.sinkhorn_iterations = 20
+    profile = Profile()
+    P, _ = cugal(copy.deepcopy(source), copy.deepcopy(target), cfg, profile)
+    assert [p.iteration_count for p in profile.sinkhorn_profiles] == [20, 20]
+    run = _oracle_run(source, target, ref, threads=16)
+    R = run.P.numpy()
+    rel = float(np.abs(P - R).max() / R.max())
+    print(f"\nbio-dmela (n = 7393), fixed iterations: max |Q - Q_FP64| / max Q = {rel:.2e}")
+    np.testing.assert_allclose(P, R, rtol=1e-4, atol=1e-6 * float(R.max()))
